@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""SpMV benchmark on one MI355X.
+
+One step = one SpMV y = A x over the whole synthetic matrix, inputs resident
+in HBM.  Before every step (warm-up and timed) a 1 GiB write evicts the
+256 MB Infinity Cache, outside the event pair, so each timed SpMV reads the
+matrix from HBM.  The step is timed with a hipEvent pair on the stream the
+kernels run on.
+
+value = algorithmic bytes / mean step time, in GB/s, where
+  B_alg = nnz*(s_v + 4) + (rows + 1)*4 + cols*s_x + rows*s_y
+(the CSR compulsory traffic; padding, COO row indices and repeated x reads
+count as overhead, not credit).  The format measured is the fastest of all
+formats on this workload (exhaustive selection, every format timed the same
+way), reported in config.format, with the per-format table alongside.
+
+cpu_baseline: the oracle's restatement of the reference CPU path
+(cpu_matrix_multiplier.cpp), timed on this host on the same matrix.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+METRIC = "SpMV GB/s & % HBM roofline per format; learned-pick slowdown vs oracle"
+
+WORKLOADS = {
+    # name: (generator workload, description)
+    "powerlaw_1m": "power-law nnz/row (Pareto density x^-2, floor, 1..50000), 1,048,576 rows, "
+                   "uniform distinct columns, values and x U[-1,1)",
+    "stencil3d_128": "3D 27-point stencil 128^3 (2,097,152 rows, 55,742,968 nnz), values N(0,1)",
+    "stencil2d_256": "2D 5-point Laplacian 256^2 (65,536 rows, 326,656 nnz)",
+}
+
+
+def b_alg(rows, cols, nnz, s_v, s_x):
+    return nnz * (s_v + 4) + (rows + 1) * 4 + cols * s_x + rows * s_x
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_plan(mfp, plan, xd, yd, reps, flush, timer):
+    ms = []
+    for _ in range(reps):
+        if flush:
+            mfp.flush_cache(1 << 30)
+        timer.start()
+        plan.spmv(xd, yd)
+        timer.stop()
+        ms.append(timer.ms())
+    return ms
+
+
+def cpu_baseline(rp, ci, va, x, nbytes, threads):
+    import oracle
+    _, t1 = oracle.spmv_seq_timed(rp, ci, va, x, reps=3)
+    times = []
+    t_end = time.time() + 8.0
+    while time.time() < t_end or len(times) < 5:
+        _, t = oracle.spmv_multi_thread(rp, ci, va, x, threads)
+        times.append(t)
+    tm = statistics.median(times)
+    return {
+        "value": round(nbytes / tm / 1e9, 3),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"full matrix, oracle restatement of cpu_csr_spmv_multi_thread_naive "
+                   f"(cpu_matrix_multiplier.cpp:229-308), {threads} threads, median of {len(times)} "
+                   f"passes = {tm * 1e3:.3f} ms; single-thread cpu_csr_spmv_single_thread_naive "
+                   f"(:20-61) mean of 3 passes = {t1 * 1e3:.3f} ms ({nbytes / t1 / 1e9:.3f} GB/s)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="powerlaw_1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--format", default="auto")
+    ap.add_argument("--select-reps", type=int, default=5)
+    ap.add_argument("--no-flush", action="store_true", help="warm-cache timing (labelled)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(the multi-GPU row split is a design, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    import matrix_format_performance_amd as mfp
+
+    dtype = np.float32 if args.dtype == "f32" else np.float64
+    s = np.dtype(dtype).itemsize
+    mfp.init(0)
+    info = mfp.device_info()
+    log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}")
+    t0 = time.time()
+    rp, ci, va = mfp.workload(args.workload, dtype)
+    rows, nnz = rp.size - 1, ci.size
+    cols = rows
+    x = mfp.generate_vector(1, cols, dtype)
+    nbytes = b_alg(rows, cols, nnz, s, s)
+    log(f"[bench] workload {args.workload}: rows={rows} nnz={nnz} B_alg={nbytes / 1e6:.1f} MB "
+        f"(generated in {time.time() - t0:.1f}s)")
+
+    A = mfp.DeviceMatrix.from_csr(rp, ci, va, cols)
+    xd = mfp.DeviceArray.from_host(x)
+    yd = mfp.DeviceArray(rows, dtype)
+    timer = mfp.Timer()
+    flush = not args.no_flush
+
+    fmts = list(mfp.FORMATS) if args.format == "auto" else [args.format]
+    table = {}
+    plans = {}
+    for f in fmts:
+        try:
+            kw = {}
+            if f in ("hyb", "hyb_atomic"):
+                kw = {"hyb_width": max(1, -(-nnz // rows))}  # tuned split: avg row length
+            p = mfp.Plan(A, f, **kw)
+        except mfp.MfpBudgetError as e:
+            table[f] = {"skipped": "layout above the memory budget"}
+            log(f"[bench] {f}: skipped ({e})")
+            continue
+        time_plan(mfp, p, xd, yd, 1, flush, timer)  # warm the code path
+        ms = time_plan(mfp, p, xd, yd, args.select_reps, flush, timer)
+        med = statistics.median(ms)
+        table[f] = {"ms": round(med, 5), "GB_per_s": round(nbytes / (med * 1e-3) / 1e9, 1),
+                    "frac_hbm": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        plans[f] = p
+        log(f"[bench] {f:13s} {med * 1e3:9.2f} us  {table[f]['GB_per_s']:8.1f} GB/s")
+    best = min(plans, key=lambda f: table[f]["ms"])
+    plan = plans[best]
+    for f, p in plans.items():
+        if f != best:
+            p.destroy()
+
+    # correctness of the measured format on this very workload
+    time_plan(mfp, plan, xd, yd, 1, flush, timer)
+    mfp.synchronize()
+    import oracle
+    y = yd.to_host()
+    y_ref = oracle.spmv_seq(rp, ci, va, x)
+    _, absdot = oracle.spmv_f64(rp, ci, va, x)
+    err = oracle.tolerance_check(y, y_ref, absdot, 1.0)
+
+    for _ in range(args.warmup):
+        time_plan(mfp, plan, xd, yd, 1, flush, timer)
+    mfp.synchronize()
+    t_wall = time.time()
+    ms = time_plan(mfp, plan, xd, yd, args.steps, flush, timer)
+    mfp.synchronize()
+    t_wall = time.time() - t_wall
+    mean_ms = statistics.fmean(ms)
+    value = nbytes / (mean_ms * 1e-3) / 1e9
+
+    traffic = None
+    tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.dtype}_{best}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mean_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "description": WORKLOADS[args.workload],
+            "rows": rows, "cols": cols, "nnz": nnz,
+            "format": best,
+            "algorithmic_bytes": nbytes,
+            "cache": "warm (no flush)" if not flush else "Infinity Cache evicted before every step",
+            "median_ms": round(statistics.median(ms), 5),
+            "min_ms": round(min(ms), 5),
+            "wall_s_timed_region_incl_flushes": round(t_wall, 4),
+            "max_norm_err_vs_cpu_reference": err,
+            "gflops": round(2 * nnz / (mean_ms * 1e-3) / 1e9, 2),
+        },
+        "pct_hbm_roofline": round(100 * value / HBM_PEAK_GBS, 2),
+        "formats": table,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(value, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(value / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(rp, ci, va, x, nbytes, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,304 @@
+/*
+ * libmfp -- MI355X (gfx950) SpMV engine, C ABI.
+ *
+ * Plain C types only: sizes, host pointers, device pointers (void*), opaque
+ * handles and an int status.  No exceptions cross this boundary.  Every
+ * enqueue-style call takes an explicit stream (NULL = the null stream) and
+ * performs no host synchronisation, so it can be captured in a HIP graph.
+ *
+ * Citations name the reference interface each entry point replaces, as
+ * path:line relative to the reference repository root.
+ */
+#ifndef MFP_H
+#define MFP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  MFP_OK = 0,
+  MFP_ERR_HIP = 1,     /* a HIP runtime call failed (text: mfp_last_error) */
+  MFP_ERR_ARG = 2,     /* invalid argument / shape */
+  MFP_ERR_BUDGET = 3,  /* the layout would exceed the memory budget */
+  MFP_ERR_STORAGE = 4, /* non-general storage (matrix_converter.cpp:19-20) */
+  MFP_ERR_NODEV = 5,   /* no usable gfx950 device */
+  MFP_ERR_STATE = 6    /* call made in the wrong state */
+};
+
+/* ---- value types ------------------------------------------------------- */
+enum { MFP_F32 = 0, MFP_F64 = 1, MFP_BF16 = 2, MFP_F16 = 3 };
+
+/* ---- formats / kernels ------------------------------------------------- */
+enum {
+  MFP_CSR_SCALAR = 0,   /* gpu_matrix_multiplier.cu:22-43   "GPU CSR"             */
+  MFP_CSR_VECTOR = 1,   /* gpu_matrix_multiplier.cu:331-362 "GPU CSR (vector)"    */
+  MFP_CSR_ADAPTIVE = 2, /* csr_adaptive_spmv.cu:26-178      "GPU CSR-Adaptive"    */
+  MFP_ELL = 3,          /* gpu_matrix_multiplier.cu:450-471 "GPU ELL"             */
+  MFP_COO = 4,          /* gpu_matrix_multiplier.cu:622-638 "GPU COO"             */
+  MFP_SCOO = 5,         /* scoo_spmv.cu:46-129              "GPU SCOO"            */
+  MFP_HYB = 6,          /* gpu_matrix_multiplier.cu:866-967 "GPU Hybrid (TODO)"   */
+  MFP_HYB_ATOMIC = 7,   /* gpu_matrix_multiplier.cu:969-1096 "GPU Hybrid (atomic)" */
+  MFP_FORMAT_COUNT = 8
+};
+
+typedef void *mfp_stream; /* hipStream_t */
+
+/* ---- runtime ----------------------------------------------------------- */
+typedef struct {
+  int device;
+  int cu_count;           /* multiProcessorCount (sm_count at main.cpp:418) */
+  int xcd_count;
+  uint64_t lds_per_block; /* sharedMemPerBlock (main.cpp:419) */
+  uint64_t free_mem, total_mem;
+  char name[256];
+  char arch[64];
+} mfp_device_info;
+
+int mfp_init(int device);
+int mfp_device_query(mfp_device_info *out);
+const char *mfp_status_string(int status);
+const char *mfp_last_error(void);
+int mfp_synchronize(void);
+int mfp_malloc(void **dev_ptr, size_t bytes);
+int mfp_free(void *dev_ptr);
+int mfp_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int mfp_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int mfp_memcpy_d2d(void *dst, const void *src, size_t bytes, mfp_stream s);
+/* dst[i] = value (value given as double, stored as dtype) */
+int mfp_fill(int dtype, void *dst, uint64_t n, double value, mfp_stream s);
+/* Overwrite `bytes` of scratch so the 256 MB Infinity Cache holds no matrix
+ * data before a timed run (pass >= 512 MB). */
+int mfp_flush_cache(uint64_t bytes, mfp_stream s);
+
+/* Event pair timing on stream s (hipEventElapsedTime, milliseconds). */
+typedef struct mfp_timer_s *mfp_timer;
+int mfp_timer_create(mfp_timer *t);
+int mfp_timer_start(mfp_timer t, mfp_stream s);
+int mfp_timer_stop(mfp_timer t, mfp_stream s);
+int mfp_timer_elapsed_ms(mfp_timer t, float *ms); /* waits for the stop event */
+int mfp_timer_destroy(mfp_timer t);
+
+/* ---- grow-only device buffer: resizable_gpu_memory<T>
+ *      (gpu/resizable_gpu_memory.h:13-44, .cpp:1-20) ------------------------ */
+typedef struct mfp_buffer_s *mfp_buffer;
+int mfp_buffer_create(mfp_buffer *b);
+int mfp_buffer_resize(mfp_buffer b, size_t bytes); /* grows only; contents lost */
+int mfp_buffer_clear(mfp_buffer b);
+void *mfp_buffer_ptr(mfp_buffer b);
+size_t mfp_buffer_size(mfp_buffer b);
+int mfp_buffer_destroy(mfp_buffer b);
+
+/* ---- device-resident matrix (CSR, u32 indices) --------------------------- */
+typedef struct mfp_matrix_s *mfp_matrix;
+typedef struct {
+  int dtype;
+  uint32_t rows, cols;
+  uint64_t nnz;
+  const uint32_t *row_ptr; /* device */
+  const uint32_t *columns; /* device */
+  const void *values;      /* device */
+} mfp_matrix_view;
+
+/* Upload a host CSR (copies). */
+int mfp_matrix_from_host_csr(mfp_matrix *out, int dtype, uint32_t rows, uint32_t cols,
+                             uint64_t nnz, const uint32_t *row_ptr,
+                             const uint32_t *columns, const void *values);
+/* Device converter: triplets (host or device arrays; values double) -> CSR on
+ * the device.  Replaces csr_matrix_class<T>(matrix_class)
+ * (common/matrix_converter.cpp:15-87); ties keep input order. */
+int mfp_matrix_from_triplets(mfp_matrix *out, int dtype, uint32_t rows, uint32_t cols,
+                             uint64_t nnz, const uint32_t *tr_rows,
+                             const uint32_t *tr_cols, const double *tr_vals,
+                             int inputs_on_device, mfp_stream s);
+int mfp_matrix_view_get(mfp_matrix m, mfp_matrix_view *out);
+int mfp_matrix_download(mfp_matrix m, uint32_t *row_ptr, uint32_t *columns, void *values);
+int mfp_matrix_destroy(mfp_matrix m);
+
+/* get_rows_statistics (common/matrix_converter.cpp:95-127), on the device. */
+typedef struct {
+  uint32_t min_elements_in_rows, max_elements_in_rows, avg_elements_in_rows;
+  double elements_in_rows_std_deviation;
+} mfp_row_stats;
+int mfp_matrix_row_stats(mfp_matrix m, mfp_row_stats *out);
+
+/* ---- plans: one format built on the device from a matrix ---------------- */
+typedef struct {
+  uint32_t lanes;         /* CSR-vector lanes per row (2..64) */
+  uint32_t nnz_per_wg;    /* CSR-Adaptive row-block size (csr_adaptive_spmv.cu:8) */
+  uint32_t long_row;      /* CSR-Adaptive: rows longer than this are split */
+  uint32_t chunk;         /* CSR-Adaptive: elements per split chunk */
+  uint32_t ell_width;     /* ELL width (0 = max row length) */
+  double hyb_percent;     /* hybrid width rule (matrix_converter.cpp:302), if hyb_width == 0 */
+  uint32_t hyb_width;     /* explicit hybrid ELL width (0 = use hyb_percent) */
+  uint32_t scoo_slices;   /* SCOO slices_count (0 = MI355X sizing) */
+  uint32_t scoo_slice_size;
+  uint32_t scoo_lanes;    /* SCOO lane_size */
+  uint32_t scoo_threads;  /* SCOO workgroup size */
+  uint64_t budget_bytes;  /* refuse layouts above this (0 = 90% of free HBM) */
+} mfp_tuning;
+
+typedef struct mfp_plan_s *mfp_plan;
+int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *out);
+int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t,
+                    mfp_stream s);
+/* y = A x on stream s (no host sync; graph-capturable). */
+int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream s);
+
+typedef struct {
+  int format, dtype;
+  uint32_t rows, cols;
+  uint64_t nnz;
+  uint64_t device_bytes;     /* format arrays owned by the plan */
+  uint64_t stored_elements;  /* ELL slots / COO entries / ... */
+  uint32_t ell_width;
+  uint64_t coo_elements;
+  uint32_t blocks, chunks;   /* CSR-Adaptive row blocks / long-row chunks */
+  uint32_t slices_count, slice_size, lane_size;
+  uint32_t lanes;
+  uint32_t nnz_per_wg;
+} mfp_plan_info;
+int mfp_plan_get_info(mfp_plan p, mfp_plan_info *out);
+
+/* Device arrays of a plan, for layout parity checks. */
+enum {
+  MFP_ARR_ELL_VALUES = 0, MFP_ARR_ELL_COLS = 1,
+  MFP_ARR_COO_VALUES = 2, MFP_ARR_COO_COLS = 3, MFP_ARR_COO_ROWS = 4,
+  MFP_ARR_SCOO_SLICES_PTR = 5, MFP_ARR_ROW_BLOCKS = 6
+};
+int mfp_plan_array(mfp_plan p, int which, const void **dev_ptr, uint64_t *count);
+int mfp_plan_destroy(mfp_plan p);
+
+/* ---- host converters run on the device (compat layer) --------------------- */
+/* fill_row_blocks (gpu/csr_adaptive_spmv.cu:180-240), nnz_per_wg generalising
+ * NNZ_PER_WG; row_ptr on the host; out gets count+1 entries (out may be NULL
+ * to query the count). */
+int mfp_fill_row_blocks(uint32_t rows, const uint32_t *row_ptr, uint32_t nnz_per_wg,
+                        uint32_t *out, uint32_t *count);
+/* scoo_matrix_class geometry (common/scoo_matrix_class.cpp:30-62). */
+int mfp_scoo_geometry(int dtype, uint32_t rows, uint32_t sm_count, uint64_t shared_mem_size,
+                      uint32_t *slices_count, uint32_t *slice_size, uint32_t *lane_size);
+
+/* ---- the reference's entry points --------------------------------------
+ * Each call does what the reference's does: grow the caller's buffers, copy
+ * the host format in, set x = 1 (and y), time the SpMV with an event pair,
+ * copy y into reusable_vector, run compare_results against reference_y
+ * (prints to stderr only; NULL skips it) and return the record.  They use
+ * this library's kernels. */
+typedef struct {
+  char format[64];
+  double elapsed;                  /* seconds */
+  double load_store_bytes;         /* the reference's byte model */
+  double operations_count;
+  double effective_bandwidth;      /* GB/s, measurement_class.cpp:13 */
+  double computational_throughput; /* GFLOP/s, measurement_class.cpp:14 */
+} mfp_measurement;
+
+typedef struct {
+  uint32_t rows, cols;
+  uint64_t nnz;
+  const uint32_t *row_ptr, *columns;
+  const void *data;
+} mfp_host_csr;
+
+typedef struct {
+  uint32_t rows, cols;
+  uint64_t nnz;
+  uint32_t elements_in_rows;
+  const uint32_t *columns;
+  const void *data;
+} mfp_host_ell;
+
+typedef struct {
+  uint32_t rows, cols;
+  uint64_t nnz;      /* matrix nnz (meta.non_zero_count) */
+  uint64_t elements; /* stored entries (get_matrix_size) */
+  const uint32_t *cols_, *rows_;
+  const void *data;
+} mfp_host_coo;
+
+typedef struct {
+  uint32_t rows, cols;
+  uint64_t nnz;
+  uint32_t slices_count, slice_size, lane_size;
+  const void *values;
+  const uint32_t *c_index, *r_index, *slices_ptr;
+} mfp_host_scoo;
+
+/* compare_results<T> (common/measurement_class.h:60-89): prints the relative
+ * squared error and the first differing entry to stderr when the error
+ * exceeds 1e-9; never fails.  dtype MFP_F32 or MFP_F64. */
+int mfp_compare_results(int dtype, uint32_t n, const void *a, const void *b);
+
+/* gpu_csr_spmv (gpu/gpu_matrix_multiplier.h:15-25, .cu:54-133) */
+int mfp_gpu_csr_spmv(int dtype, const mfp_host_csr *A, mfp_buffer A_dev, mfp_buffer col_ids,
+                     mfp_buffer row_ptr, mfp_buffer x, mfp_buffer y, void *reusable_vector,
+                     const void *reference_y, mfp_measurement *out);
+/* gpu_csr_vector_spmv (gpu/gpu_matrix_multiplier.h:39-49, .cu:364-447) */
+int mfp_gpu_csr_vector_spmv(int dtype, const mfp_host_csr *A, mfp_buffer A_dev,
+                            mfp_buffer col_ids, mfp_buffer row_ptr, mfp_buffer x,
+                            mfp_buffer y, void *reusable_vector, const void *reference_y,
+                            mfp_measurement *out);
+/* gpu_csr_adaptive_spmv (gpu/csr_adaptive_spmv.h:13-23, .cu:242-321) */
+int mfp_gpu_csr_adaptive_spmv(int dtype, const mfp_host_csr *A, mfp_buffer A_dev,
+                              mfp_buffer col_ids, mfp_buffer row_ptr, mfp_buffer x,
+                              mfp_buffer y, void *reusable_vector, const void *reference_y,
+                              mfp_measurement *out);
+/* gpu_ell_spmv (gpu/gpu_matrix_multiplier.h:64-73, .cu:473-550) */
+int mfp_gpu_ell_spmv(int dtype, const mfp_host_ell *A, mfp_buffer A_dev, mfp_buffer col_ids,
+                     mfp_buffer x, mfp_buffer y, void *reusable_vector,
+                     const void *reference_y, mfp_measurement *out);
+/* gpu_coo_spmv (gpu/gpu_matrix_multiplier.h:87-97, .cu:640-716) */
+int mfp_gpu_coo_spmv(int dtype, const mfp_host_coo *A, mfp_buffer A_dev, mfp_buffer col_ids,
+                     mfp_buffer row_ids, mfp_buffer x, mfp_buffer y, void *reusable_vector,
+                     const void *reference_y, mfp_measurement *out);
+/* gpu_scoo_spmv (gpu/scoo_spmv.h:12-23, .cu:144-222) */
+int mfp_gpu_scoo_spmv(int dtype, int print_diff, const mfp_host_scoo *A, mfp_buffer A_dev,
+                      mfp_buffer r_index, mfp_buffer c_index, mfp_buffer x, mfp_buffer y,
+                      void *reusable_vector, const void *reference_y, mfp_measurement *out);
+/* gpu_hybrid_spmv (gpu/gpu_matrix_multiplier.h:112-124, .cu:866-967) */
+int mfp_gpu_hybrid_spmv(int dtype, const mfp_host_ell *ell, const mfp_host_coo *coo,
+                        mfp_buffer A_ell, mfp_buffer A_coo, mfp_buffer ell_col_ids,
+                        mfp_buffer coo_col_ids, mfp_buffer coo_row_ids, mfp_buffer x,
+                        mfp_buffer y, void *reusable_vector, const void *reference_y,
+                        mfp_measurement *out);
+/* gpu_hybrid_atomic_spmv (gpu/gpu_matrix_multiplier.h:126-138, .cu:1004-1096) */
+int mfp_gpu_hybrid_atomic_spmv(int dtype, const mfp_host_ell *ell, const mfp_host_coo *coo,
+                               mfp_buffer A_ell, mfp_buffer A_coo, mfp_buffer ell_col_ids,
+                               mfp_buffer coo_col_ids, mfp_buffer coo_row_ids, mfp_buffer x,
+                               mfp_buffer y, void *reusable_vector, const void *reference_y,
+                               mfp_measurement *out);
+
+/* ---- synthetic matrices (host generator, seeded, deterministic) ----------- */
+enum {
+  MFP_GEN_STENCIL2D = 0, /* 5-point, n x n grid */
+  MFP_GEN_STENCIL3D = 1, /* 27-point, n x n x n grid */
+  MFP_GEN_POWERLAW = 2,  /* nnz/row = min(max, floor(1/U)) (Pareto density x^-2) */
+  MFP_GEN_UNIFORM = 3    /* nnz/row uniform in [lo, hi] */
+};
+enum { MFP_VAL_LAPLACE = 0, MFP_VAL_UNIFORM = 1, MFP_VAL_NORMAL = 2 };
+typedef struct {
+  int family;
+  int values;            /* MFP_VAL_* */
+  uint64_t seed;
+  uint32_t n;            /* stencil grid edge */
+  uint32_t rows;         /* powerlaw / uniform */
+  uint32_t nnz_lo, nnz_hi; /* powerlaw min/max row length, uniform range */
+  double alpha;          /* powerlaw density exponent (2.0 uses the exact 1/U form) */
+  double bandwidth;      /* column window as a fraction of cols (1 = full) */
+} mfp_gen_params;
+/* Pass NULL arrays to get rows/nnz first. */
+int mfp_generate(const mfp_gen_params *p, uint32_t *rows, uint32_t *cols, uint64_t *nnz,
+                 uint32_t *row_ptr, uint32_t *columns, void *values, int dtype);
+/* x ~ U[-1,1) from seed (host). */
+int mfp_generate_vector(uint64_t seed, uint64_t n, void *x, int dtype);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

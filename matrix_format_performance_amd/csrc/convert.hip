@@ -1,0 +1,637 @@
+// Device-side matrix converters.
+//
+// The reference builds every layout on the host (common/matrix_converter.cpp,
+// common/scoo_matrix_class.cpp, gpu/csr_adaptive_spmv.cu:180-240).  Here each
+// layout is built by HIP kernels in HBM; the results are bit-identical to the
+// reference's host algorithms (ties in triplets->CSR keep input order).
+#include "mfp_internal.h"
+#include "wave64.h"
+
+#include <algorithm>
+#include <cmath>
+#include <utility>
+#include <vector>
+
+namespace mfp {
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of u32 (hierarchical: tile scan, scan of tile sums, add).
+// ---------------------------------------------------------------------------
+constexpr uint32_t SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t *sh, uint32_t *total)
+{
+  // Hillis-Steele over 256 thread values in LDS.
+  const uint32_t t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < SCAN_T; off <<= 1) {
+    uint32_t add = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  uint32_t incl = sh[t];
+  if (total) *total = sh[SCAN_T - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const uint32_t *__restrict__ in,
+                                                       uint32_t *__restrict__ out, uint64_t n,
+                                                       uint32_t *__restrict__ sums)
+{
+  __shared__ uint32_t tile[SCAN_TILE];
+  __shared__ uint32_t sh[SCAN_T];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t k = 0; k < SCAN_I; k++) {
+    uint64_t i = base + k * SCAN_T + t;
+    tile[k * SCAN_T + t] = i < n ? in[i] : 0u;
+  }
+  __syncthreads();
+  uint32_t loc[SCAN_I];
+  uint32_t acc = 0;
+  for (uint32_t k = 0; k < SCAN_I; k++) {
+    loc[k] = acc;
+    acc += tile[t * SCAN_I + k];
+  }
+  uint32_t total = 0;
+  uint32_t pre = block_exclusive_scan_256(acc, sh, &total);
+  for (uint32_t k = 0; k < SCAN_I; k++) tile[t * SCAN_I + k] = loc[k] + pre;
+  __syncthreads();
+  for (uint32_t k = 0; k < SCAN_I; k++) {
+    uint64_t i = base + k * SCAN_T + t;
+    if (i < n) out[i] = tile[k * SCAN_T + t];
+  }
+  if (t == 0 && sums) sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(uint32_t *__restrict__ out, uint64_t n, const uint32_t *__restrict__ sums)
+{
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  const uint32_t add = sums[blockIdx.x];
+  for (uint32_t k = 0; k < SCAN_I; k++) {
+    uint64_t i = base + k * SCAN_T + threadIdx.x;
+    if (i < n) out[i] += add;
+  }
+}
+
+__global__ void k_scan_total(const uint32_t *in, const uint32_t *out, uint64_t n, uint32_t *total)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *total = n ? out[n - 1] + in[n - 1] : 0u;
+}
+
+static int scan_rec(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s)
+{
+  if (n == 0) return MFP_OK;
+  const uint64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  uint32_t *sums = nullptr;
+  if (tiles > 1) MFP_CHECK(dmalloc((void **)&sums, tiles * 4));
+  k_scan_tiles<<<(uint32_t)tiles, SCAN_T, 0, s>>>(in, out, n, sums);
+  MFP_HIP(hipGetLastError());
+  if (tiles > 1) {
+    int st = scan_rec(sums, sums, tiles, s);
+    if (st == MFP_OK) {
+      k_scan_add<<<(uint32_t)tiles, SCAN_T, 0, s>>>(out, n, sums);
+      st = hipGetLastError() == hipSuccess ? MFP_OK : MFP_ERR_HIP;
+    }
+    MFP_HIP(hipStreamSynchronize(s)); // sums is freed below
+    dfree(sums);
+    return st;
+  }
+  return MFP_OK;
+}
+
+// In-place (in == out) is allowed.  total_out (host) gets in[0..n) summed.
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s,
+                       uint32_t *total_out)
+{
+  uint32_t last_in = 0;
+  if (total_out && n) MFP_HIP(hipMemcpyAsync(&last_in, in + n - 1, 4, hipMemcpyDeviceToHost, s));
+  if (total_out) MFP_HIP(hipStreamSynchronize(s));
+  MFP_CHECK(scan_rec(in, out, n, s));
+  if (total_out) {
+    uint32_t last_out = 0;
+    if (n) MFP_HIP(hipMemcpyAsync(&last_out, out + n - 1, 4, hipMemcpyDeviceToHost, s));
+    MFP_HIP(hipStreamSynchronize(s));
+    *total_out = n ? last_out + last_in : 0;
+  }
+  return MFP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Triplets -> CSR: counts + scan give row_ptr; a stable LSD radix sort of
+// (row, col, input index) by col then row gives the element order.
+// ---------------------------------------------------------------------------
+__global__ void k_count_rows(const uint32_t *__restrict__ r, const uint32_t *__restrict__ c,
+                             uint64_t n, uint32_t rows, uint32_t cols,
+                             uint32_t *__restrict__ cnt, uint32_t *__restrict__ bad)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint32_t rr = r[i], cc = c[i];
+    if (rr >= rows || cc >= cols) {
+      atomicOr(bad, 1u);
+      continue;
+    }
+    atomicAdd(cnt + rr, 1u);
+  }
+}
+
+__global__ void k_iota(uint32_t *__restrict__ p, uint64_t n)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = (uint32_t)i;
+}
+
+constexpr uint32_t RX_T = 256, RX_I = 8, RX_TILE = RX_T * RX_I;
+
+__global__ __launch_bounds__(RX_T) void k_radix_hist(const uint32_t *__restrict__ key, uint64_t n,
+                                                     uint32_t shift, uint32_t *__restrict__ hist,
+                                                     uint32_t ntiles)
+{
+  __shared__ uint32_t cnt[256];
+  const uint32_t t = threadIdx.x;
+  cnt[t] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RX_TILE;
+  for (uint32_t k = 0; k < RX_I; k++) {
+    uint64_t i = base + k * RX_T + t;
+    if (i < n) atomicAdd(&cnt[(key[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)t * ntiles + blockIdx.x] = cnt[t];
+}
+
+// Stable scatter of three u32 arrays by the 8-bit digit of key.  Elements are
+// ranked in tile order (round k = elements k*256 .. k*256+255, by thread).
+__global__ __launch_bounds__(RX_T) void k_radix_scatter(
+    const uint32_t *__restrict__ key, const uint32_t *__restrict__ a0,
+    const uint32_t *__restrict__ a1, const uint32_t *__restrict__ a2, uint32_t *__restrict__ o0,
+    uint32_t *__restrict__ o1, uint32_t *__restrict__ o2, uint64_t n, uint32_t shift,
+    const uint32_t *__restrict__ offs, uint32_t ntiles)
+{
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[RX_T / 64][256];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  run[t] = offs[(uint64_t)t * ntiles + blockIdx.x];
+  const uint64_t base = (uint64_t)blockIdx.x * RX_TILE;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : ((1ull << lane) - 1ull);
+  for (uint32_t k = 0; k < RX_I; k++) {
+    for (uint32_t w = 0; w < RX_T / 64; w++) wcnt[w][t] = 0;
+    __syncthreads();
+    const uint64_t i = base + k * RX_T + t;
+    const bool valid = i < n;
+    const uint32_t d = valid ? (key[i] >> shift) & 255u : 0u;
+    uint64_t peers = __ballot(valid);
+    for (uint32_t b = 0; b < 8; b++) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = __popcll(peers & lt_mask);
+    if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pre = 0;
+      for (uint32_t w = 0; w < wave; w++) pre += wcnt[w][d];
+      const uint32_t pos = run[d] + pre + rank;
+      o0[pos] = a0[i];
+      o1[pos] = a1[i];
+      o2[pos] = a2[i];
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < RX_T / 64; w++) tot += wcnt[w][t];
+    run[t] += tot;
+  }
+}
+
+template <typename T>
+__global__ void k_gather_values(const uint32_t *__restrict__ idx, const double *__restrict__ v,
+                                T *__restrict__ out, uint64_t n)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = (T)v[idx[i]];
+}
+
+static uint32_t bit_length(uint32_t v)
+{
+  uint32_t b = 0;
+  while (v) {
+    b++;
+    v >>= 1;
+  }
+  return b;
+}
+
+static uint32_t grid_for(uint64_t n, uint32_t bs = 256)
+{
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + bs - 1) / bs, 1u << 16));
+}
+
+int csr_from_triplets_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz,
+                             const uint32_t *tr_rows, const uint32_t *tr_cols,
+                             const double *tr_vals, uint32_t *row_ptr, uint32_t *columns,
+                             void *values, hipStream_t s)
+{
+  MFP_ARG(nnz < (1ull << 32), "nnz must fit in 32 bits");
+  uint32_t *bad = nullptr;
+  MFP_CHECK(dmalloc((void **)&bad, 4));
+  MFP_CHECK(fill_u32(bad, 0, 1, s));
+  MFP_CHECK(fill_u32(row_ptr, 0, (uint64_t)rows + 1, s));
+  if (nnz) {
+    k_count_rows<<<grid_for(nnz), 256, 0, s>>>(tr_rows, tr_cols, nnz, rows, cols, row_ptr, bad);
+    MFP_HIP(hipGetLastError());
+  }
+  uint32_t hbad = 0;
+  MFP_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  dfree(bad);
+  if (hbad) {
+    set_error("triplet row/col index out of range");
+    return MFP_ERR_ARG;
+  }
+  MFP_CHECK(exclusive_scan_u32(row_ptr, row_ptr, (uint64_t)rows + 1, s, nullptr));
+  if (nnz == 0) return MFP_OK;
+
+  // ping-pong buffers: (row, col, idx)
+  uint32_t *buf[6] = {};
+  for (int i = 0; i < 6; i++) {
+    int st = dmalloc((void **)&buf[i], nnz * 4);
+    if (st != MFP_OK) {
+      for (int j = 0; j < i; j++) dfree(buf[j]);
+      return st;
+    }
+  }
+  const uint32_t ntiles = (uint32_t)((nnz + RX_TILE - 1) / RX_TILE);
+  uint32_t *hist = nullptr;
+  int st = dmalloc((void **)&hist, (uint64_t)ntiles * 256 * 4);
+  auto cleanup = [&]() {
+    for (int i = 0; i < 6; i++) dfree(buf[i]);
+    dfree(hist);
+  };
+  if (st != MFP_OK) {
+    cleanup();
+    return st;
+  }
+  // own copies of the keys, so the caller's arrays are never written
+  MFP_HIP(hipMemcpyAsync(buf[0], tr_rows, nnz * 4, hipMemcpyDeviceToDevice, s));
+  MFP_HIP(hipMemcpyAsync(buf[1], tr_cols, nnz * 4, hipMemcpyDeviceToDevice, s));
+  k_iota<<<grid_for(nnz), 256, 0, s>>>(buf[2], nnz);
+  uint32_t *cur[3] = {buf[0], buf[1], buf[2]};
+  uint32_t *nxt[3] = {buf[3], buf[4], buf[5]};
+  // passes: col digits (LSD), then row digits
+  std::vector<std::pair<int, uint32_t>> passes;
+  for (uint32_t sh = 0; sh < bit_length(cols ? cols - 1 : 0); sh += 8) passes.push_back({1, sh});
+  for (uint32_t sh = 0; sh < bit_length(rows ? rows - 1 : 0); sh += 8) passes.push_back({0, sh});
+  for (auto [which, sh] : passes) {
+    const uint32_t *key = cur[which];
+    k_radix_hist<<<ntiles, RX_T, 0, s>>>(key, nnz, sh, hist, ntiles);
+    if (hipGetLastError() != hipSuccess) {
+      cleanup();
+      set_error("radix hist launch failed");
+      return MFP_ERR_HIP;
+    }
+    st = exclusive_scan_u32(hist, hist, (uint64_t)ntiles * 256, s, nullptr);
+    if (st != MFP_OK) {
+      cleanup();
+      return st;
+    }
+    k_radix_scatter<<<ntiles, RX_T, 0, s>>>(key, cur[0], cur[1], cur[2], nxt[0], nxt[1], nxt[2],
+                                             nnz, sh, hist, ntiles);
+    if (hipGetLastError() != hipSuccess) {
+      cleanup();
+      set_error("radix scatter launch failed");
+      return MFP_ERR_HIP;
+    }
+    std::swap(cur, nxt);
+  }
+  // cur[1] = sorted columns, cur[2] = source index
+  MFP_HIP(hipMemcpyAsync(columns, cur[1], nnz * 4, hipMemcpyDeviceToDevice, s));
+  if (dtype == MFP_F32)
+    k_gather_values<float><<<grid_for(nnz), 256, 0, s>>>(cur[2], tr_vals, (float *)values, nnz);
+  else
+    k_gather_values<double><<<grid_for(nnz), 256, 0, s>>>(cur[2], tr_vals, (double *)values, nnz);
+  MFP_HIP(hipGetLastError());
+  MFP_HIP(hipStreamSynchronize(s));
+  cleanup();
+  return MFP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Row statistics (get_rows_statistics, common/matrix_converter.cpp:95-127).
+// min/max/sum exactly; sum of squared deviations from the integer average in
+// u64 (exact; equals the reference's double sum whenever that sum is below
+// 2^53, where every partial double sum is exact).
+// ---------------------------------------------------------------------------
+struct stats_acc {
+  uint32_t mn, mx;
+  unsigned long long sum, sq;
+};
+
+__global__ void k_row_minmax(const uint32_t *__restrict__ rp, uint32_t rows, stats_acc *acc)
+{
+  uint32_t mn = 0xFFFFFFFFu, mx = 0;
+  unsigned long long sum = 0;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < rows; i += stride) {
+    uint32_t len = rp[i + 1] - rp[i];
+    mn = min(mn, len);
+    mx = max(mx, len);
+    sum += len;
+  }
+  atomicMin(&acc->mn, mn);
+  atomicMax(&acc->mx, mx);
+  atomicAdd(&acc->sum, sum);
+}
+
+__global__ void k_row_sq(const uint32_t *__restrict__ rp, uint32_t rows, stats_acc *acc)
+{
+  const uint32_t avg = rows ? (uint32_t)acc->sum / rows : 0u; // 32-bit sum as the reference
+  unsigned long long sq = 0;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < rows; i += stride) {
+    long long d = (long long)(rp[i + 1] - rp[i]) - (long long)avg;
+    sq += (unsigned long long)(d * d);
+  }
+  atomicAdd(&acc->sq, sq);
+}
+
+int row_stats_device(uint32_t rows, const uint32_t *rp, mfp_row_stats *out, hipStream_t s)
+{
+  stats_acc init{0xFFFFFFFFu - 1u, 0u, 0ull, 0ull}; // min starts at UINT_MAX-1 (:98)
+  stats_acc *acc = nullptr;
+  MFP_CHECK(dmalloc((void **)&acc, sizeof(stats_acc)));
+  MFP_HIP(hipMemcpyAsync(acc, &init, sizeof(init), hipMemcpyHostToDevice, s));
+  const uint32_t g = std::min<uint32_t>(grid_for(rows), 2048);
+  if (rows) {
+    k_row_minmax<<<g, 256, 0, s>>>(rp, rows, acc);
+    k_row_sq<<<g, 256, 0, s>>>(rp, rows, acc);
+    MFP_HIP(hipGetLastError());
+  }
+  stats_acc h{};
+  MFP_HIP(hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  dfree(acc);
+  out->min_elements_in_rows = h.mn;
+  out->max_elements_in_rows = h.mx;
+  out->avg_elements_in_rows = rows ? (uint32_t)h.sum / rows : 0u;
+  out->elements_in_rows_std_deviation = rows ? std::sqrt((double)h.sq / rows) : 0.0;
+  return MFP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ELL (common/matrix_converter.cpp:139-203): data[row + k*rows], pad (0, 0),
+// rows longer than the width truncated.  Every slot is written.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_ell_build(uint32_t rows, const uint32_t *__restrict__ rp,
+                            const uint32_t *__restrict__ ci, const T *__restrict__ a,
+                            uint32_t width, T *__restrict__ ev, uint32_t *__restrict__ ec)
+{
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const uint32_t s = rp[r], len = rp[r + 1] - s;
+  for (uint32_t k = 0; k < width; k++) {
+    const uint64_t o = (uint64_t)r + (uint64_t)k * rows;
+    if (k < len) {
+      ev[o] = a[s + k];
+      ec[o] = ci[s + k];
+    } else {
+      ev[o] = (T)0;
+      ec[o] = 0u;
+    }
+  }
+}
+
+int ell_build_device(int dtype, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
+                     const void *val, uint32_t width, void *ev, uint32_t *ec, hipStream_t s)
+{
+  if (rows == 0 || width == 0) return MFP_OK;
+  const uint32_t g = ceil_div_u32(rows, 256);
+  if (dtype == MFP_F32)
+    k_ell_build<float><<<g, 256, 0, s>>>(rows, rp, ci, (const float *)val, width, (float *)ev, ec);
+  else
+    k_ell_build<double><<<g, 256, 0, s>>>(rows, rp, ci, (const double *)val, width, (double *)ev, ec);
+  MFP_HIP(hipGetLastError());
+  return MFP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// COO (common/matrix_converter.cpp:211-280): CSR order, entries whose in-row
+// offset is >= element_start.  Element-parallel with a binary search for the
+// row of each CSR element.
+// ---------------------------------------------------------------------------
+__global__ void k_tail_counts(uint32_t rows, const uint32_t *__restrict__ rp, uint32_t e0,
+                              uint32_t *__restrict__ cnt)
+{
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const uint32_t len = rp[r + 1] - rp[r];
+  cnt[r] = len > e0 ? len - e0 : 0u;
+}
+
+__device__ __forceinline__ uint32_t row_of(const uint32_t *__restrict__ rp, uint32_t rows, uint32_t e)
+{
+  // largest r with rp[r] <= e (and rp[r+1] > e)
+  uint32_t lo = 0, hi = rows; // invariant: rp[lo] <= e < rp[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (rp[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <typename T>
+__global__ void k_coo_build(uint32_t rows, uint64_t nnz, const uint32_t *__restrict__ rp,
+                            const uint32_t *__restrict__ ci, const T *__restrict__ a,
+                            uint32_t e0, const uint32_t *__restrict__ toff,
+                            T *__restrict__ cv, uint32_t *__restrict__ cc,
+                            uint32_t *__restrict__ cr)
+{
+  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; e < nnz; e += stride) {
+    const uint32_t r = row_of(rp, rows, (uint32_t)e);
+    const uint32_t k = (uint32_t)e - rp[r];
+    if (k < e0) continue;
+    const uint32_t o = toff[r] + (k - e0);
+    cv[o] = a[e];
+    cc[o] = ci[e];
+    cr[o] = r;
+  }
+}
+
+int coo_tail_build_device(int dtype, uint32_t rows, uint64_t nnz, const uint32_t *rp,
+                          const uint32_t *ci, const void *val, uint32_t e0, void **cv,
+                          uint32_t **cc, uint32_t **cr, uint64_t *count, hipStream_t s)
+{
+  *cv = nullptr;
+  *cc = nullptr;
+  *cr = nullptr;
+  uint32_t *toff = nullptr;
+  MFP_CHECK(dmalloc((void **)&toff, ((uint64_t)rows + 1) * 4));
+  uint32_t total = 0;
+  if (rows) {
+    k_tail_counts<<<ceil_div_u32(rows, 256), 256, 0, s>>>(rows, rp, e0, toff);
+    MFP_HIP(hipGetLastError());
+  }
+  int st = exclusive_scan_u32(toff, toff, rows, s, &total);
+  if (st != MFP_OK) {
+    dfree(toff);
+    return st;
+  }
+  *count = total;
+  const size_t vs = dtype_size(dtype);
+  st = dmalloc(cv, (uint64_t)total * vs);
+  if (st == MFP_OK) st = dmalloc((void **)cc, (uint64_t)total * 4);
+  if (st == MFP_OK) st = dmalloc((void **)cr, (uint64_t)total * 4);
+  if (st != MFP_OK) {
+    dfree(toff);
+    return st;
+  }
+  if (total) {
+    if (dtype == MFP_F32)
+      k_coo_build<float><<<grid_for(nnz), 256, 0, s>>>(rows, nnz, rp, ci, (const float *)val, e0,
+                                                       toff, (float *)*cv, *cc, *cr);
+    else
+      k_coo_build<double><<<grid_for(nnz), 256, 0, s>>>(rows, nnz, rp, ci, (const double *)val,
+                                                        e0, toff, (double *)*cv, *cc, *cr);
+    MFP_HIP(hipGetLastError());
+  }
+  MFP_HIP(hipStreamSynchronize(s));
+  dfree(toff);
+  return MFP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SCOO slice pointers (common/scoo_matrix_class.cpp:74-100): slice s starts
+// at row s*slice_size; entry [real slices] = nnz; the rest stay 0.
+// ---------------------------------------------------------------------------
+__global__ void k_scoo_slices(uint32_t rows, uint64_t nnz, const uint32_t *__restrict__ rp,
+                              uint32_t slices_count, uint32_t ss, uint32_t *__restrict__ sp)
+{
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > slices_count) return;
+  const uint32_t real = rows ? (uint32_t)(((uint64_t)rows + ss - 1) / ss) : 0u;
+  sp[s] = s < real ? rp[(uint64_t)s * ss] : (s == real ? (uint32_t)nnz : 0u);
+}
+
+int scoo_slices_device(uint32_t rows, uint64_t nnz, const uint32_t *rp, uint32_t slices_count,
+                       uint32_t ss, uint32_t *sp, hipStream_t s)
+{
+  MFP_ARG(ss > 0, "slice_size must be positive");
+  k_scoo_slices<<<ceil_div_u32((uint64_t)slices_count + 1, 256), 256, 0, s>>>(rows, nnz, rp,
+                                                                             slices_count, ss, sp);
+  MFP_HIP(hipGetLastError());
+  return MFP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Row blocks (fill_row_blocks, gpu/csr_adaptive_spmv.cu:180-240) in parallel.
+// The greedy partition is a chain 0 -> close(0) -> ...: close(s) is the row
+// where a block started at s closes (or NONE when the loop ends first), and it
+// depends on s only.  Chain nodes are found by pointer jumping: mark[0] = 1;
+// round k marks J_k(s) for every marked s with J_k = close^(2^k).  Marked
+// nodes in [0, rows] in order, followed by rows, are the reference's output.
+// ---------------------------------------------------------------------------
+__global__ void k_close(uint32_t rows, const uint32_t *__restrict__ rp, uint32_t W,
+                        uint32_t *__restrict__ J)
+{
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > (uint64_t)rows + 1) return;
+  const uint32_t NONE = rows + 1;
+  if (s >= rows) {
+    J[s] = NONE;
+    return;
+  }
+  uint32_t sum = 0;
+  uint32_t res = NONE;
+  for (uint64_t i = s + 1; i <= rows; i++) {
+    sum += rp[i] - rp[i - 1];
+    if (sum == W) {
+      res = (uint32_t)i;
+      break;
+    }
+    if (sum > W) {
+      res = (i - s > 1) ? (uint32_t)(i - 1) : (uint32_t)i;
+      break;
+    }
+    if (i - s > W) {
+      res = (uint32_t)i;
+      break;
+    }
+  }
+  J[s] = res;
+}
+
+__global__ void k_mark(uint64_t n, const uint32_t *__restrict__ J, uint32_t *__restrict__ mark)
+{
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  if (mark[s]) mark[J[s]] = 1u;
+}
+
+__global__ void k_jump(uint64_t n, const uint32_t *__restrict__ J, uint32_t *__restrict__ J2)
+{
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  J2[s] = J[J[s]];
+}
+
+__global__ void k_compact(uint32_t rows, const uint32_t *__restrict__ mark,
+                          const uint32_t *__restrict__ pos, uint32_t *__restrict__ out)
+{
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > rows) return;
+  if (mark[s]) out[pos[s]] = (uint32_t)s;
+}
+
+__global__ void k_store1(uint32_t *p, uint32_t v) { *p = v; }
+
+int row_blocks_device(uint32_t rows, const uint32_t *rp, uint32_t W, uint32_t **blocks_out,
+                      uint32_t *count_out, hipStream_t s)
+{
+  MFP_ARG(W > 0, "nnz_per_wg must be positive");
+  const uint64_t n = (uint64_t)rows + 2; // nodes 0..rows, sentinel rows+1
+  uint32_t *J = nullptr, *J2 = nullptr, *mark = nullptr;
+  MFP_CHECK(dmalloc((void **)&J, n * 4));
+  MFP_CHECK(dmalloc((void **)&J2, n * 4));
+  MFP_CHECK(dmalloc((void **)&mark, n * 4));
+  const uint32_t g = ceil_div_u32(n, 256);
+  k_close<<<g, 256, 0, s>>>(rows, rp, W, J);
+  MFP_CHECK(fill_u32(mark, 0, n, s));
+  MFP_CHECK(fill_u32(mark, 1, 1, s));
+  for (uint64_t span = 1; span < 2 * n; span <<= 1) {
+    k_mark<<<g, 256, 0, s>>>(n, J, mark);
+    k_jump<<<g, 256, 0, s>>>(n, J, J2);
+    std::swap(J, J2);
+  }
+  MFP_HIP(hipGetLastError());
+  // count = marked nodes in [0, rows]; positions by exclusive scan
+  uint32_t count = 0;
+  MFP_CHECK(exclusive_scan_u32(mark, J2, (uint64_t)rows + 1, s, &count));
+  uint32_t *out = nullptr;
+  MFP_CHECK(dmalloc((void **)&out, ((uint64_t)count + 1) * 4));
+  k_compact<<<ceil_div_u32((uint64_t)rows + 1, 256), 256, 0, s>>>(rows, mark, J2, out);
+  k_store1<<<1, 1, 0, s>>>(out + count, rows);
+  MFP_HIP(hipGetLastError());
+  MFP_HIP(hipStreamSynchronize(s));
+  dfree(J);
+  dfree(J2);
+  dfree(mark);
+  *blocks_out = out;
+  *count_out = count;
+  return MFP_OK;
+}
+
+} // namespace mfp

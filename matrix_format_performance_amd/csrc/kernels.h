@@ -1,0 +1,60 @@
+// Kernel launchers (spmv.hip) shared by plans and the reference entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mfp {
+
+struct adaptive_args {
+  uint32_t rows = 0;
+  const uint32_t *rp = nullptr, *ci = nullptr;
+  const uint32_t *blocks = nullptr; // nblocks + 1 row boundaries
+  uint32_t nblocks = 0;
+  uint32_t nnz_per_wg = 0;
+  uint32_t long_row = 0;
+  const uint4 *chunks = nullptr; // {row, piece, first item, pieces}
+  uint32_t nchunks = 0;
+  uint32_t chunk = 0;
+  void *partials = nullptr; // nchunks values
+  uint32_t *tickets = nullptr; // nchunks counters, zero between calls
+};
+
+int launch_csr_scalar(int dtype, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
+                      const void *a, const void *x, void *y, hipStream_t s);
+int launch_csr_vector(int dtype, uint32_t lanes, uint32_t rows, const uint32_t *rp,
+                      const uint32_t *ci, const void *a, const void *x, void *y, hipStream_t s);
+int launch_csr_adaptive(int dtype, const adaptive_args &g, const void *a, const void *x, void *y,
+                        hipStream_t s);
+int launch_ell(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const void *ev,
+               const void *x, void *y, hipStream_t s);
+int launch_coo_add(int dtype, uint64_t n, const uint32_t *cr, const uint32_t *cc, const void *cv,
+                   const void *x, void *y, hipStream_t s);
+int launch_hyb_atomic(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const void *ev,
+                      uint64_t ncoo, const uint32_t *cr, const uint32_t *cc, const void *cv,
+                      const void *x, void *y, hipStream_t s);
+int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t threads,
+                const uint32_t *cr, const uint32_t *cc, const void *cv, const uint32_t *sp,
+                const void *x, void *y, hipStream_t s);
+
+// Row blocks + long-row chunk list for CSR-Adaptive (plan.hip).  The arrays
+// are owned by the caller afterwards (free with dfree).
+struct adaptive_layout {
+  uint32_t *blocks = nullptr;
+  uint32_t nblocks = 0;
+  uint4 *chunks = nullptr;
+  uint32_t nchunks = 0;
+  void *partials = nullptr;
+  uint32_t *tickets = nullptr;
+};
+int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t nnz_per_wg,
+                          uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s);
+void free_adaptive_layout(adaptive_layout *l);
+
+int csr_from_triplets_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz,
+                             const uint32_t *tr_rows, const uint32_t *tr_cols,
+                             const double *tr_vals, uint32_t *row_ptr, uint32_t *columns,
+                             void *values, hipStream_t s);
+
+} // namespace mfp

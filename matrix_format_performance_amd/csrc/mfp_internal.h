@@ -1,0 +1,98 @@
+// Internal declarations shared by the libmfp translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/mfp.h"
+
+namespace mfp {
+
+// Per-thread last error text, returned by mfp_last_error().
+void set_error(const std::string &msg);
+const std::string &last_error();
+
+// Return from the calling C-ABI function with MFP_ERR_HIP when a HIP call fails.
+#define MFP_HIP(expr)                                                           \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      ::mfp::set_error(std::string(#expr) + ": " + hipGetErrorString(e_) +      \
+                       " (" __FILE__ ":" + std::to_string(__LINE__) + ")");     \
+      return MFP_ERR_HIP;                                                       \
+    }                                                                           \
+  } while (0)
+
+#define MFP_CHECK(expr)                                                         \
+  do {                                                                          \
+    int s_ = (expr);                                                            \
+    if (s_ != MFP_OK)                                                           \
+      return s_;                                                                \
+  } while (0)
+
+#define MFP_ARG(cond, msg)                                                      \
+  do {                                                                          \
+    if (!(cond)) {                                                              \
+      ::mfp::set_error(std::string("invalid argument: ") + (msg));              \
+      return MFP_ERR_ARG;                                                       \
+    }                                                                           \
+  } while (0)
+
+inline size_t dtype_size(int dtype)
+{
+  switch (dtype) {
+  case MFP_F32: return 4;
+  case MFP_F64: return 8;
+  case MFP_BF16: return 2;
+  case MFP_F16: return 2;
+  default: return 0;
+  }
+}
+
+inline hipStream_t as_stream(mfp_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline uint32_t ceil_div_u32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Device properties cached by mfp_init.
+struct device_props {
+  int device = -1;
+  int cu_count = 0;
+  int xcd_count = 8;
+  size_t lds_per_block = 0;
+  size_t free_mem = 0, total_mem = 0;
+  char name[256] = {0};
+  char arch[64] = {0};
+};
+const device_props &props();
+int ensure_init();
+
+// Stream-ordered device allocation helpers (plain hipMalloc; plans own them).
+int dmalloc(void **p, size_t bytes);
+void dfree(void *p);
+
+// ---- converter entry points used by plans (convert.hip) -------------------
+int row_stats_device(uint32_t rows, const uint32_t *row_ptr_d, mfp_row_stats *out,
+                     hipStream_t s);
+int ell_build_device(int dtype, uint32_t rows, const uint32_t *row_ptr, const uint32_t *col,
+                     const void *val, uint32_t width, void *ell_val, uint32_t *ell_col,
+                     hipStream_t s);
+int coo_tail_build_device(int dtype, uint32_t rows, uint64_t nnz, const uint32_t *row_ptr,
+                          const uint32_t *col, const void *val, uint32_t element_start,
+                          void **coo_val, uint32_t **coo_col, uint32_t **coo_row,
+                          uint64_t *count, hipStream_t s);
+int scoo_slices_device(uint32_t rows, uint64_t nnz, const uint32_t *row_ptr,
+                       uint32_t slices_count, uint32_t slice_size, uint32_t *slices_ptr,
+                       hipStream_t s);
+int row_blocks_device(uint32_t rows, const uint32_t *row_ptr, uint32_t nnz_per_wg,
+                      uint32_t **blocks_out, uint32_t *count_out, hipStream_t s);
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s,
+                       uint32_t *total_out /* host, may be null */);
+int fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+int fill_bytes_zero(void *p, size_t bytes, hipStream_t s);
+int convert_values(int dtype_in, const void *in, int dtype_out, void *out, uint64_t n,
+                   hipStream_t s);
+
+} // namespace mfp

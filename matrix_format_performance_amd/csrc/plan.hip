@@ -1,0 +1,575 @@
+// Device-resident matrices and per-format plans.
+#include "kernels.h"
+#include "mfp_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace mfp {
+
+// ---- CSR-Adaptive layout: row blocks + long-row chunk items ----------------
+__global__ void k_chunk_counts(const uint32_t *__restrict__ blocks, uint32_t nb,
+                               const uint32_t *__restrict__ rp, uint32_t long_row,
+                               uint32_t chunk, uint32_t *__restrict__ cnt)
+{
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const uint32_t r0 = blocks[b], r1 = blocks[b + 1];
+  uint32_t c = 0;
+  if (r1 == r0 + 1) {
+    const uint32_t len = rp[r0 + 1] - rp[r0];
+    if (len > long_row) c = (len + chunk - 1) / chunk;
+  }
+  cnt[b] = c;
+}
+
+__global__ void k_chunk_fill(const uint32_t *__restrict__ blocks, uint32_t nb,
+                             const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ base,
+                             uint4 *__restrict__ chunks)
+{
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const uint32_t n = cnt[b];
+  for (uint32_t c = 0; c < n; c++) chunks[base[b] + c] = make_uint4(blocks[b], c, base[b], n);
+}
+
+void free_adaptive_layout(adaptive_layout *l)
+{
+  dfree(l->blocks);
+  dfree(l->chunks);
+  dfree(l->partials);
+  dfree(l->tickets);
+  *l = adaptive_layout{};
+}
+
+int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t W,
+                          uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s)
+{
+  MFP_ARG(W >= 1 && W <= 16384, "nnz_per_wg must be in [1, 16384]");
+  MFP_ARG(chunk >= 256, "chunk must be >= 256");
+  MFP_ARG(long_row >= chunk, "long_row must be >= chunk");
+  adaptive_layout l;
+  MFP_CHECK(row_blocks_device(rows, rp, W, &l.blocks, &l.nblocks, s));
+  uint32_t *cnt = nullptr, *base = nullptr;
+  int st = dmalloc((void **)&cnt, (uint64_t)l.nblocks * 4 + 4);
+  if (st == MFP_OK) st = dmalloc((void **)&base, (uint64_t)l.nblocks * 4 + 4);
+  if (st == MFP_OK && l.nblocks) {
+    k_chunk_counts<<<ceil_div_u32(l.nblocks, 256), 256, 0, s>>>(l.blocks, l.nblocks, rp, long_row,
+                                                                chunk, cnt);
+    st = exclusive_scan_u32(cnt, base, l.nblocks, s, &l.nchunks);
+  }
+  if (st == MFP_OK) st = dmalloc((void **)&l.chunks, (uint64_t)l.nchunks * sizeof(uint4));
+  if (st == MFP_OK) st = dmalloc(&l.partials, (uint64_t)l.nchunks * dtype_size(dtype));
+  if (st == MFP_OK) st = dmalloc((void **)&l.tickets, (uint64_t)l.nchunks * 4);
+  if (st == MFP_OK && l.nchunks) {
+    k_chunk_fill<<<ceil_div_u32(l.nblocks, 256), 256, 0, s>>>(l.blocks, l.nblocks, cnt, base, l.chunks);
+    st = fill_u32(l.tickets, 0, l.nchunks, s);
+  }
+  if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
+  dfree(cnt);
+  dfree(base);
+  if (st != MFP_OK) {
+    free_adaptive_layout(&l);
+    return st;
+  }
+  *out = l;
+  return MFP_OK;
+}
+
+} // namespace mfp
+
+using namespace mfp;
+
+struct mfp_matrix_s {
+  int dtype = MFP_F32;
+  uint32_t rows = 0, cols = 0;
+  uint64_t nnz = 0;
+  uint32_t *row_ptr = nullptr, *columns = nullptr;
+  void *values = nullptr;
+  bool stats_valid = false;
+  mfp_row_stats stats{};
+};
+
+struct mfp_plan_s {
+  int format = 0, dtype = 0;
+  mfp_matrix m = nullptr;
+  mfp_tuning t{};
+  uint32_t ell_w = 0;
+  void *ell_val = nullptr;
+  uint32_t *ell_col = nullptr;
+  uint64_t ncoo = 0;
+  void *coo_val = nullptr;
+  uint32_t *coo_col = nullptr, *coo_row = nullptr;
+  uint32_t slices = 0, ss = 0, L = 0, scoo_threads = 512;
+  uint32_t *sp = nullptr;
+  adaptive_layout ad;
+  uint64_t device_bytes = 0;
+};
+
+static int matrix_stats(mfp_matrix m, mfp_row_stats *out, hipStream_t s)
+{
+  if (!m->stats_valid) {
+    MFP_CHECK(row_stats_device(m->rows, m->row_ptr, &m->stats, s));
+    m->stats_valid = true;
+  }
+  *out = m->stats;
+  return MFP_OK;
+}
+
+static uint32_t next_pow2_u32(uint32_t v)
+{
+  uint32_t p = 1;
+  while (p < v && p < 0x80000000u) p <<= 1;
+  return p;
+}
+
+static uint32_t prev_pow2_u32(uint32_t v)
+{
+  if (v == 0) return 0;
+  uint32_t p = 1;
+  while ((p << 1) <= v && (p << 1) != 0) p <<= 1;
+  return p;
+}
+
+// MI355X SCOO sizing: a slice's accumulator acc[slice_size][lane_size] takes
+// 32 KB of the 160 KB LDS (4 resident workgroups per CU), with at least
+// 4 slices per CU when the row count allows.
+static void scoo_mi355x_geometry(int dtype, uint32_t rows, uint32_t *slices, uint32_t *ss,
+                                 uint32_t *L)
+{
+  const uint32_t cus = (uint32_t)std::max(1, props().cu_count);
+  const uint32_t acc_elems = (uint32_t)((32u * 1024u) / dtype_size(dtype));
+  uint32_t s = prev_pow2_u32(std::max<uint32_t>(1, rows / (4 * cus)));
+  s = std::min<uint32_t>(std::max<uint32_t>(s, 64), acc_elems / 2);
+  uint32_t lanes = std::min<uint32_t>(32, prev_pow2_u32(acc_elems / s));
+  *ss = s;
+  *L = std::max<uint32_t>(1, lanes);
+  *slices = rows ? ceil_div_u32(rows, s) : 1;
+}
+
+static int plan_free_arrays(mfp_plan p)
+{
+  dfree(p->ell_val);
+  dfree(p->ell_col);
+  dfree(p->coo_val);
+  dfree(p->coo_col);
+  dfree(p->coo_row);
+  dfree(p->sp);
+  free_adaptive_layout(&p->ad);
+  return MFP_OK;
+}
+
+static uint64_t budget_of(const mfp_tuning &t)
+{
+  if (t.budget_bytes) return t.budget_bytes;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+  return (uint64_t)(0.9 * (double)fr);
+}
+
+extern "C" {
+
+int mfp_matrix_from_host_csr(mfp_matrix *out, int dtype, uint32_t rows, uint32_t cols,
+                             uint64_t nnz, const uint32_t *row_ptr, const uint32_t *columns,
+                             const void *values)
+{
+  MFP_ARG(out && row_ptr, "null pointer");
+  MFP_ARG(dtype == MFP_F32 || dtype == MFP_F64, "dtype must be f32 or f64");
+  MFP_ARG(nnz < (1ull << 32), "nnz must fit in 32 bits");
+  MFP_ARG(row_ptr[0] == 0 && row_ptr[rows] == nnz, "row_ptr must start at 0 and end at nnz");
+  MFP_CHECK(ensure_init());
+  auto *m = new mfp_matrix_s;
+  m->dtype = dtype;
+  m->rows = rows;
+  m->cols = cols;
+  m->nnz = nnz;
+  const size_t vs = dtype_size(dtype);
+  int st = dmalloc((void **)&m->row_ptr, ((uint64_t)rows + 1) * 4);
+  if (st == MFP_OK) st = dmalloc((void **)&m->columns, nnz * 4);
+  if (st == MFP_OK) st = dmalloc(&m->values, nnz * vs);
+  if (st == MFP_OK) st = mfp_memcpy_h2d(m->row_ptr, row_ptr, ((uint64_t)rows + 1) * 4);
+  if (st == MFP_OK) st = mfp_memcpy_h2d(m->columns, columns, nnz * 4);
+  if (st == MFP_OK) st = mfp_memcpy_h2d(m->values, values, nnz * vs);
+  if (st != MFP_OK) {
+    mfp_matrix_destroy(m);
+    return st;
+  }
+  *out = m;
+  return MFP_OK;
+}
+
+int mfp_matrix_from_triplets(mfp_matrix *out, int dtype, uint32_t rows, uint32_t cols,
+                             uint64_t nnz, const uint32_t *tr_rows, const uint32_t *tr_cols,
+                             const double *tr_vals, int inputs_on_device, mfp_stream sv)
+{
+  MFP_ARG(out, "null out");
+  MFP_ARG(dtype == MFP_F32 || dtype == MFP_F64, "dtype must be f32 or f64");
+  MFP_ARG(nnz < (1ull << 32), "nnz must fit in 32 bits");
+  MFP_CHECK(ensure_init());
+  hipStream_t s = as_stream(sv);
+  const uint32_t *dr = tr_rows, *dc = tr_cols;
+  const double *dv = tr_vals;
+  void *tmp[3] = {nullptr, nullptr, nullptr};
+  auto release = [&]() {
+    for (void *p : tmp) dfree(p);
+  };
+  if (!inputs_on_device && nnz) {
+    int st = dmalloc(&tmp[0], nnz * 4);
+    if (st == MFP_OK) st = dmalloc(&tmp[1], nnz * 4);
+    if (st == MFP_OK) st = dmalloc(&tmp[2], nnz * 8);
+    if (st == MFP_OK) st = mfp_memcpy_h2d(tmp[0], tr_rows, nnz * 4);
+    if (st == MFP_OK) st = mfp_memcpy_h2d(tmp[1], tr_cols, nnz * 4);
+    if (st == MFP_OK) st = mfp_memcpy_h2d(tmp[2], tr_vals, nnz * 8);
+    if (st != MFP_OK) {
+      release();
+      return st;
+    }
+    dr = (const uint32_t *)tmp[0];
+    dc = (const uint32_t *)tmp[1];
+    dv = (const double *)tmp[2];
+  }
+  auto *m = new mfp_matrix_s;
+  m->dtype = dtype;
+  m->rows = rows;
+  m->cols = cols;
+  m->nnz = nnz;
+  int st = dmalloc((void **)&m->row_ptr, ((uint64_t)rows + 1) * 4);
+  if (st == MFP_OK) st = dmalloc((void **)&m->columns, nnz * 4);
+  if (st == MFP_OK) st = dmalloc(&m->values, nnz * dtype_size(dtype));
+  if (st == MFP_OK)
+    st = csr_from_triplets_device(dtype, rows, cols, nnz, dr, dc, dv, m->row_ptr, m->columns,
+                                  m->values, s);
+  if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
+  release();
+  if (st != MFP_OK) {
+    mfp_matrix_destroy(m);
+    return st;
+  }
+  *out = m;
+  return MFP_OK;
+}
+
+int mfp_matrix_view_get(mfp_matrix m, mfp_matrix_view *out)
+{
+  MFP_ARG(m && out, "null pointer");
+  out->dtype = m->dtype;
+  out->rows = m->rows;
+  out->cols = m->cols;
+  out->nnz = m->nnz;
+  out->row_ptr = m->row_ptr;
+  out->columns = m->columns;
+  out->values = m->values;
+  return MFP_OK;
+}
+
+int mfp_matrix_download(mfp_matrix m, uint32_t *row_ptr, uint32_t *columns, void *values)
+{
+  MFP_ARG(m, "null matrix");
+  if (row_ptr) MFP_CHECK(mfp_memcpy_d2h(row_ptr, m->row_ptr, ((uint64_t)m->rows + 1) * 4));
+  if (columns) MFP_CHECK(mfp_memcpy_d2h(columns, m->columns, m->nnz * 4));
+  if (values) MFP_CHECK(mfp_memcpy_d2h(values, m->values, m->nnz * dtype_size(m->dtype)));
+  return MFP_OK;
+}
+
+int mfp_matrix_destroy(mfp_matrix m)
+{
+  if (!m) return MFP_OK;
+  dfree(m->row_ptr);
+  dfree(m->columns);
+  dfree(m->values);
+  delete m;
+  return MFP_OK;
+}
+
+int mfp_matrix_row_stats(mfp_matrix m, mfp_row_stats *out)
+{
+  MFP_ARG(m && out, "null pointer");
+  MFP_CHECK(ensure_init());
+  return matrix_stats(m, out, nullptr);
+}
+
+int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *t)
+{
+  MFP_ARG(m && t, "null pointer");
+  MFP_ARG(format >= 0 && format < MFP_FORMAT_COUNT, "format");
+  MFP_CHECK(ensure_init());
+  std::memset(t, 0, sizeof(*t));
+  const uint32_t avg = m->rows ? (uint32_t)((m->nnz + m->rows - 1) / m->rows) : 1;
+  t->lanes = std::min<uint32_t>(64, std::max<uint32_t>(2, next_pow2_u32(std::max<uint32_t>(1, avg))));
+  t->nnz_per_wg = 1024;
+  t->chunk = 2048;
+  t->long_row = 4096;
+  t->ell_width = 0;
+  t->hyb_percent = 0.2; // main.cpp:240
+  t->hyb_width = 0;
+  scoo_mi355x_geometry(m->dtype, m->rows, &t->scoo_slices, &t->scoo_slice_size, &t->scoo_lanes);
+  t->scoo_threads = 512;
+  t->budget_bytes = 0;
+  (void)format;
+  return MFP_OK;
+}
+
+int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *tin, mfp_stream sv)
+{
+  MFP_ARG(out && m, "null pointer");
+  MFP_ARG(format >= 0 && format < MFP_FORMAT_COUNT, "format");
+  MFP_CHECK(ensure_init());
+  hipStream_t s = as_stream(sv);
+  mfp_tuning t;
+  if (tin)
+    t = *tin;
+  else
+    MFP_CHECK(mfp_tuning_default(m, format, &t));
+  auto *p = new mfp_plan_s;
+  p->format = format;
+  p->dtype = m->dtype;
+  p->m = m;
+  p->t = t;
+  const size_t vs = dtype_size(m->dtype);
+  int st = MFP_OK;
+  auto fail = [&](int code) {
+    plan_free_arrays(p);
+    delete p;
+    return code;
+  };
+  switch (format) {
+  case MFP_CSR_SCALAR: break;
+  case MFP_CSR_VECTOR:
+    if (t.lanes < 2 || t.lanes > 64 || (t.lanes & (t.lanes - 1))) {
+      set_error("lanes must be a power of two in [2, 64]");
+      return fail(MFP_ERR_ARG);
+    }
+    break;
+  case MFP_CSR_ADAPTIVE:
+    st = build_adaptive_layout(m->dtype, m->rows, m->row_ptr, t.nnz_per_wg, t.long_row, t.chunk,
+                               &p->ad, s);
+    if (st != MFP_OK) return fail(st);
+    p->device_bytes = ((uint64_t)p->ad.nblocks + 1) * 4 + (uint64_t)p->ad.nchunks * (16 + vs + 4);
+    break;
+  case MFP_ELL:
+  case MFP_HYB:
+  case MFP_HYB_ATOMIC: {
+    mfp_row_stats rs;
+    st = matrix_stats(m, &rs, s);
+    if (st != MFP_OK) return fail(st);
+    uint32_t w;
+    if (format == MFP_ELL) {
+      w = t.ell_width ? t.ell_width : rs.max_elements_in_rows;
+      if (t.ell_width && t.ell_width < rs.max_elements_in_rows) {
+        set_error("ELL width below the longest row would drop entries; use the hybrid format");
+        return fail(MFP_ERR_ARG);
+      }
+    } else if (t.hyb_width) {
+      w = t.hyb_width;
+    } else {
+      // hybrid_matrix_class::allocate (common/matrix_converter.cpp:296-302)
+      const double wd = (double)rs.avg_elements_in_rows +
+                        (double)(rs.max_elements_in_rows - rs.avg_elements_in_rows) * t.hyb_percent;
+      w = (uint32_t)wd;
+    }
+    const uint64_t slots = (uint64_t)m->rows * w;
+    const uint64_t need = slots * (vs + 4);
+    if (need > budget_of(t)) {
+      set_error("ELL part needs " + std::to_string(need) + " bytes, above the budget");
+      return fail(MFP_ERR_BUDGET);
+    }
+    p->ell_w = w;
+    st = dmalloc(&p->ell_val, slots * vs);
+    if (st == MFP_OK) st = dmalloc((void **)&p->ell_col, slots * 4);
+    if (st == MFP_OK)
+      st = ell_build_device(m->dtype, m->rows, m->row_ptr, m->columns, m->values, w, p->ell_val,
+                            p->ell_col, s);
+    if (st != MFP_OK) return fail(st);
+    p->device_bytes = need;
+    if (format != MFP_ELL) {
+      st = coo_tail_build_device(m->dtype, m->rows, m->nnz, m->row_ptr, m->columns, m->values, w,
+                                 &p->coo_val, &p->coo_col, &p->coo_row, &p->ncoo, s);
+      if (st != MFP_OK) return fail(st);
+      p->device_bytes += p->ncoo * (vs + 8);
+    }
+    break;
+  }
+  case MFP_COO:
+  case MFP_SCOO: {
+    st = coo_tail_build_device(m->dtype, m->rows, m->nnz, m->row_ptr, m->columns, m->values, 0,
+                               &p->coo_val, &p->coo_col, &p->coo_row, &p->ncoo, s);
+    if (st != MFP_OK) return fail(st);
+    p->device_bytes = p->ncoo * (vs + 8);
+    if (format == MFP_SCOO) {
+      p->ss = t.scoo_slice_size;
+      p->L = t.scoo_lanes;
+      p->slices = t.scoo_slices;
+      p->scoo_threads = t.scoo_threads ? t.scoo_threads : 512;
+      if (p->ss == 0 || p->L == 0 || (p->L & (p->L - 1)) ||
+          (m->rows && (uint64_t)p->slices * p->ss < m->rows)) {
+        set_error("SCOO geometry must cover every row with a power-of-two lane size");
+        return fail(MFP_ERR_ARG);
+      }
+      st = dmalloc((void **)&p->sp, ((uint64_t)p->slices + 1) * 4);
+      if (st == MFP_OK) st = scoo_slices_device(m->rows, m->nnz, m->row_ptr, p->slices, p->ss, p->sp, s);
+      if (st != MFP_OK) return fail(st);
+      p->device_bytes += ((uint64_t)p->slices + 1) * 4;
+    }
+    break;
+  }
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("plan construction failed");
+    return fail(MFP_ERR_HIP);
+  }
+  *out = p;
+  return MFP_OK;
+}
+
+int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
+{
+  MFP_ARG(p && x && y, "null pointer");
+  hipStream_t s = as_stream(sv);
+  const mfp_matrix m = p->m;
+  const int dt = p->dtype;
+  switch (p->format) {
+  case MFP_CSR_SCALAR:
+    return launch_csr_scalar(dt, m->rows, m->row_ptr, m->columns, m->values, x, y, s);
+  case MFP_CSR_VECTOR:
+    return launch_csr_vector(dt, p->t.lanes, m->rows, m->row_ptr, m->columns, m->values, x, y, s);
+  case MFP_CSR_ADAPTIVE: {
+    adaptive_args g;
+    g.rows = m->rows;
+    g.rp = m->row_ptr;
+    g.ci = m->columns;
+    g.blocks = p->ad.blocks;
+    g.nblocks = p->ad.nblocks;
+    g.nnz_per_wg = p->t.nnz_per_wg;
+    g.long_row = p->t.long_row;
+    g.chunks = p->ad.chunks;
+    g.nchunks = p->ad.nchunks;
+    g.chunk = p->t.chunk;
+    g.partials = p->ad.partials;
+    g.tickets = p->ad.tickets;
+    return launch_csr_adaptive(dt, g, m->values, x, y, s);
+  }
+  case MFP_ELL:
+    return launch_ell(dt, m->rows, p->ell_w, p->ell_col, p->ell_val, x, y, s);
+  case MFP_COO:
+    MFP_CHECK(fill_bytes_zero(y, (size_t)m->rows * dtype_size(dt), s));
+    return launch_coo_add(dt, p->ncoo, p->coo_row, p->coo_col, p->coo_val, x, y, s);
+  case MFP_SCOO:
+    return launch_scoo(dt, m->rows, p->ss, p->L, p->scoo_threads, p->coo_row, p->coo_col,
+                       p->coo_val, p->sp, x, y, s);
+  case MFP_HYB:
+    MFP_CHECK(launch_ell(dt, m->rows, p->ell_w, p->ell_col, p->ell_val, x, y, s));
+    return launch_coo_add(dt, p->ncoo, p->coo_row, p->coo_col, p->coo_val, x, y, s);
+  case MFP_HYB_ATOMIC:
+    MFP_CHECK(fill_bytes_zero(y, (size_t)m->rows * dtype_size(dt), s));
+    return launch_hyb_atomic(dt, m->rows, p->ell_w, p->ell_col, p->ell_val, p->ncoo, p->coo_row,
+                             p->coo_col, p->coo_val, x, y, s);
+  }
+  set_error("unknown format");
+  return MFP_ERR_ARG;
+}
+
+int mfp_plan_get_info(mfp_plan p, mfp_plan_info *o)
+{
+  MFP_ARG(p && o, "null pointer");
+  std::memset(o, 0, sizeof(*o));
+  o->format = p->format;
+  o->dtype = p->dtype;
+  o->rows = p->m->rows;
+  o->cols = p->m->cols;
+  o->nnz = p->m->nnz;
+  o->device_bytes = p->device_bytes;
+  o->ell_width = p->ell_w;
+  o->coo_elements = p->ncoo;
+  o->stored_elements = (uint64_t)p->m->rows * p->ell_w + p->ncoo;
+  o->blocks = p->ad.nblocks;
+  o->chunks = p->ad.nchunks;
+  o->slices_count = p->slices;
+  o->slice_size = p->ss;
+  o->lane_size = p->L;
+  o->lanes = p->t.lanes;
+  o->nnz_per_wg = p->t.nnz_per_wg;
+  return MFP_OK;
+}
+
+int mfp_plan_array(mfp_plan p, int which, const void **ptr, uint64_t *count)
+{
+  MFP_ARG(p && ptr && count, "null pointer");
+  const uint64_t slots = (uint64_t)p->m->rows * p->ell_w;
+  switch (which) {
+  case MFP_ARR_ELL_VALUES: *ptr = p->ell_val; *count = slots; break;
+  case MFP_ARR_ELL_COLS: *ptr = p->ell_col; *count = slots; break;
+  case MFP_ARR_COO_VALUES: *ptr = p->coo_val; *count = p->ncoo; break;
+  case MFP_ARR_COO_COLS: *ptr = p->coo_col; *count = p->ncoo; break;
+  case MFP_ARR_COO_ROWS: *ptr = p->coo_row; *count = p->ncoo; break;
+  case MFP_ARR_SCOO_SLICES_PTR: *ptr = p->sp; *count = p->sp ? (uint64_t)p->slices + 1 : 0; break;
+  case MFP_ARR_ROW_BLOCKS: *ptr = p->ad.blocks; *count = p->ad.blocks ? (uint64_t)p->ad.nblocks + 1 : 0; break;
+  default: MFP_ARG(false, "array id");
+  }
+  return MFP_OK;
+}
+
+int mfp_plan_destroy(mfp_plan p)
+{
+  if (!p) return MFP_OK;
+  plan_free_arrays(p);
+  delete p;
+  return MFP_OK;
+}
+
+int mfp_fill_row_blocks(uint32_t rows, const uint32_t *row_ptr, uint32_t W, uint32_t *out,
+                        uint32_t *count)
+{
+  MFP_ARG(row_ptr && count, "null pointer");
+  MFP_CHECK(ensure_init());
+  uint32_t *drp = nullptr, *blocks = nullptr;
+  MFP_CHECK(dmalloc((void **)&drp, ((uint64_t)rows + 1) * 4));
+  int st = mfp_memcpy_h2d(drp, row_ptr, ((uint64_t)rows + 1) * 4);
+  uint32_t n = 0;
+  if (st == MFP_OK) st = row_blocks_device(rows, drp, W, &blocks, &n, nullptr);
+  if (st == MFP_OK && out) st = mfp_memcpy_d2h(out, blocks, ((uint64_t)n + 1) * 4);
+  dfree(drp);
+  dfree(blocks);
+  if (st == MFP_OK) *count = n;
+  return st;
+}
+
+// scoo_matrix_class sizing (common/scoo_matrix_class.cpp:13-62), restated.
+int mfp_scoo_geometry(int dtype, uint32_t rows, uint32_t sm_count, uint64_t shm,
+                      uint32_t *slices_count, uint32_t *slice_size, uint32_t *lane_size)
+{
+  MFP_ARG(slices_count && slice_size && lane_size, "null pointer");
+  MFP_ARG(sm_count > 0, "sm_count must be positive");
+  MFP_ARG(dtype == MFP_F32 || dtype == MFP_F64, "dtype");
+  auto next_pow = [](uint32_t x) {
+    --x;
+    x |= x >> 1u;
+    x |= x >> 2u;
+    x |= x >> 4u;
+    x |= x >> 8u;
+    x |= x >> 16u;
+    return ++x;
+  };
+  auto prev_pow = [&](uint32_t x) {
+    const uint32_t n = next_pow(x);
+    return n > x ? n / 2 : n;
+  };
+  const uint64_t vs = dtype_size(dtype);
+  const uint32_t max_rows = (uint32_t)std::min<uint64_t>(512, shm / vs);
+  uint32_t mult = 1;
+  while (max_rows < rows / (sm_count * mult)) mult++;
+  const uint32_t slices = sm_count * mult;
+  const uint32_t ss = (rows + slices - 1) / slices;
+  uint32_t lane = 0;
+  if (ss) {
+    const uint32_t inner = prev_pow((uint32_t)(shm / ((uint64_t)ss * vs)));
+    lane = prev_pow(std::min(slices, inner));
+    while (ss * lane > 1024) lane /= 2;
+  }
+  *slices_count = slices;
+  *slice_size = ss;
+  *lane_size = lane ? lane : 1;
+  return MFP_OK;
+}
+
+} // extern "C"

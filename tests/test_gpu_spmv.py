@@ -1,0 +1,143 @@
+"""GPU parity: every format's SpMV against the oracle.
+
+Deterministic kernels (CSR-scalar, CSR-vector, CSR-Adaptive, ELL) must be
+bit-exact against the oracle's emulator of their summation order.  Every
+format must match the reference CPU answer (cpu_matrix_multiplier.cpp:20-61,
+sequential) within 1e-5 (FP32) / 1e-12 (FP64) per row, relative to
+sum_j |a_ij x_j|.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gpu_cases import CASES, announce, case
+
+pytestmark = pytest.mark.gpu
+
+RTOL = {np.dtype(np.float32): 1e-5, np.dtype(np.float64): 1e-12}
+DTYPES = ["float32", "float64"]
+
+
+def _run(mfp, name, dtype, fmt, **tuning):
+    rp, ci, va, cols = case(name, dtype)
+    announce(f"spmv fmt={fmt} case={name} dtype={dtype} rows={rp.size - 1} nnz={ci.size} tuning={tuning}")
+    x = mfp.generate_vector(7, cols, np.dtype(dtype))
+    A = mfp.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = mfp.Plan(A, fmt, **tuning)
+    xd = mfp.DeviceArray.from_host(x)
+    yd = mfp.DeviceArray(rp.size - 1, np.dtype(dtype))
+    yd.fill(np.nan)  # any row left unwritten shows as NaN
+    plan.spmv(xd, yd)
+    mfp.synchronize()
+    y = yd.to_host()
+    info = plan.info()
+    plan.destroy()
+    A.destroy()
+    return y, (rp, ci, va, x), info
+
+
+def _tol_check(y, rp, ci, va, x):
+    y_ref = oracle.spmv_seq(rp, ci, va, x)
+    y64, absdot = oracle.spmv_f64(rp, ci, va, x)
+    assert not np.isnan(y).any(), "unwritten rows"
+    rtol = RTOL[va.dtype]
+    assert oracle.tolerance_check(y, y_ref, absdot, rtol) <= rtol
+    assert oracle.tolerance_check(y, y64, absdot, rtol) <= rtol
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("name", CASES)
+def test_csr_scalar_bit_exact(gpu, name, dtype):
+    y, (rp, ci, va, x), _ = _run(gpu, name, dtype, "csr_scalar")
+    assert np.array_equal(y, oracle.spmv_seq(rp, ci, va, x))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("name", ["stencil3d_20", "powerlaw_20k", "edge_rows"])
+def test_csr_vector_bit_exact(gpu, name, dtype, lanes):
+    y, (rp, ci, va, x), _ = _run(gpu, name, dtype, "csr_vector", lanes=lanes)
+    assert np.array_equal(y, oracle.spmv_lanes(rp, ci, va, x, lanes))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("name", CASES)
+def test_csr_adaptive_bit_exact_default(gpu, name, dtype):
+    y, (rp, ci, va, x), info = _run(gpu, name, dtype, "csr_adaptive")
+    ref = oracle.spmv_adaptive(rp, ci, va, x, info["nnz_per_wg"], 256, 4096, 2048)
+    assert np.array_equal(y, ref)
+    _tol_check(y, rp, ci, va, x)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("W,long_row,chunk", [(64, 256, 256), (256, 512, 256), (4096, 8192, 4096), (2048, 1024, 512)])
+@pytest.mark.parametrize("name", ["powerlaw_20k", "edge_rows", "uniform_band"])
+def test_csr_adaptive_bit_exact_tunings(gpu, name, dtype, W, long_row, chunk):
+    if long_row < chunk:
+        pytest.skip("long_row must be >= chunk")
+    y, (rp, ci, va, x), info = _run(gpu, name, dtype, "csr_adaptive", nnz_per_wg=W, long_row=long_row,
+                                    chunk=chunk)
+    blocks = oracle.fill_row_blocks(rp, W).astype(np.int64)
+    lens = np.diff(rp.astype(np.int64))
+    expect_chunks = sum(-(-int(lens[s]) // chunk) for s, e in zip(blocks[:-1], blocks[1:])
+                        if e - s == 1 and lens[s] > long_row)
+    assert info["blocks"] == blocks.size - 1 and info["chunks"] == expect_chunks
+    assert np.array_equal(y, oracle.spmv_adaptive(rp, ci, va, x, W, 256, long_row, chunk))
+
+
+def test_csr_adaptive_repeated_calls_reset_tickets(gpu):
+    # the long-row ticket counters must be back at zero after each call
+    rp, ci, va, cols = case("powerlaw_20k", "float32")
+    announce("adaptive repeated calls powerlaw_20k")
+    x = gpu.generate_vector(5, cols, np.float32)
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "csr_adaptive", nnz_per_wg=256, long_row=512, chunk=256)
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(rp.size - 1, np.float32)
+    ref = oracle.spmv_adaptive(rp, ci, va, x, 256, 256, 512, 256)
+    for _ in range(5):
+        yd.fill(np.nan)
+        plan.spmv(xd, yd)
+        gpu.synchronize()
+        assert np.array_equal(yd.to_host(), ref)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("name", ["stencil2d_64", "stencil3d_20", "uniform_band", "edge_rows", "one_by_one"])
+def test_ell_bit_exact(gpu, name, dtype):
+    y, (rp, ci, va, x), info = _run(gpu, name, dtype, "ell")
+    w, ed, ec = oracle.ell_from_csr(rp, ci, va)
+    assert info["ell_width"] == w
+    assert np.array_equal(y, oracle.spmv_ell(rp.size - 1, w, ec, ed, x))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("fmt", ["coo", "scoo", "hyb", "hyb_atomic"])
+@pytest.mark.parametrize("name", CASES)
+def test_atomic_formats_within_tolerance(gpu, name, dtype, fmt):
+    tuning = {}
+    if fmt in ("hyb", "hyb_atomic") and name in ("powerlaw_20k", "edge_rows"):
+        tuning = {"hyb_width": 8}  # the 20% rule would make the ELL part huge
+    y, (rp, ci, va, x), _ = _run(gpu, name, dtype, fmt, **tuning)
+    _tol_check(y, rp, ci, va, x)
+
+
+@pytest.mark.parametrize("ss,lanes,threads", [(64, 32, 256), (256, 4, 512), (1024, 8, 1024), (4096, 2, 512)])
+def test_scoo_geometries(gpu, ss, lanes, threads):
+    rp, ci, va, cols = case("uniform_band", "float32")
+    rows = rp.size - 1
+    slices = -(-rows // ss)
+    y, (rp, ci, va, x), info = _run(gpu, "uniform_band", "float32", "scoo", scoo_slices=slices,
+                                    scoo_slice_size=ss, scoo_lanes=lanes, scoo_threads=threads)
+    assert info["slices_count"] == slices
+    _tol_check(y, rp, ci, va, x)
+
+
+def test_ell_budget_refusal(gpu):
+    # a width * rows layout above the budget is refused, not attempted
+    rp, ci, va, cols = case("powerlaw_20k", "float32")
+    announce("ell budget refusal")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    with pytest.raises(gpu.MfpBudgetError):
+        gpu.Plan(A, "ell", budget_bytes=1 << 20)
+    A.destroy()

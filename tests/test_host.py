@@ -1,0 +1,191 @@
+"""CPU-only checks of libmfp's host side: the C ABI loads and exports every
+declared symbol, the seeded generator, and the reference-interface records
+(measurement_class, compare_results) against the reference's own sources."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import matrix_format_performance_amd as mfp
+from matrix_format_performance_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(mfp_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(L.LIB_PATH)
+    names = _declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes table covers the same set
+    assert sorted(L.SIGNATURES) == names
+
+
+def test_no_gpu_calls_fail_cleanly():
+    # without a device every entry point returns a status; nothing crashes
+    import ctypes
+    lib = L.load()
+    p = ctypes.c_void_p()
+    st = lib.mfp_malloc(ctypes.byref(p), 16)
+    assert st in (L.MFP_OK, L.MFP_ERR_NODEV, L.MFP_ERR_HIP)
+    if st == L.MFP_OK:
+        lib.mfp_free(p)
+
+
+def test_package_fails_loudly_without_library(tmp_path):
+    # a copy of the package without libmfp.so must raise on first use
+    import shutil
+    dst = tmp_path / "matrix_format_performance_amd"
+    shutil.copytree(os.path.join(ROOT, "matrix_format_performance_amd"), dst,
+                    ignore=shutil.ignore_patterns("*.so", "_build", "__pycache__"))
+    code = ("import matrix_format_performance_amd as m\n"
+            "try:\n m.generate_vector(0, 4)\nexcept ImportError as e:\n print('IMPORTERROR')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=tmp_path, capture_output=True, text=True)
+    assert "IMPORTERROR" in r.stdout, r.stdout + r.stderr
+
+
+# ---- generator ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64])
+def test_stencil2d_closed_form(n):
+    rp, ci, va = mfp.generate(mfp.GenSpec("stencil2d", values="laplace", n=n))
+    assert rp.size - 1 == n * n and ci.size == 5 * n * n - 4 * n
+    x = np.ones(n * n, np.float32)
+    # row sums: 4 - (#neighbours) = 0 interior, 1 edges, 2 corners
+    rows = np.repeat(np.arange(n * n), np.diff(rp))
+    y = np.zeros(n * n)
+    np.add.at(y, rows, va.astype(np.float64) * x[ci])
+    grid = y.reshape(n, n)
+    if n >= 3:
+        assert grid[1:-1, 1:-1].max() == 0 and grid[0, 1] == 1 and grid[0, 0] == 2
+
+
+@pytest.mark.parametrize("n", [1, 3, 10])
+def test_stencil3d_closed_form(n):
+    rp, ci, va = mfp.generate(mfp.GenSpec("stencil3d", values="normal", n=n, seed=3))
+    assert rp.size - 1 == n ** 3 and ci.size == (3 * n - 2) ** 3
+    for r in range(0, n ** 3, max(1, n ** 3 // 50)):
+        seg = ci[rp[r]:rp[r + 1]].astype(np.int64)
+        assert np.all(np.diff(seg) > 0) and r in seg
+
+
+def test_full_size_nnz_counts():
+    # the benchmark configurations: 2D 256^2 and 3D 128^3 (closed forms)
+    rp, ci, _ = mfp.workload("stencil2d_256")
+    assert ci.size == 326_656
+    rp, ci, _ = mfp.workload("stencil3d_128")
+    assert ci.size == 55_742_968
+
+
+def test_powerlaw_rows_distinct_sorted_and_deterministic():
+    spec = mfp.GenSpec("powerlaw", values="uniform", rows=20000, nnz_lo=1, nnz_hi=5000, seed=9)
+    rp, ci, va = mfp.generate(spec)
+    rp2, ci2, va2 = mfp.generate(spec)
+    assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2) and np.array_equal(va, va2)
+    lens = np.diff(rp.astype(np.int64))
+    assert lens.min() >= 1 and lens.max() <= 5000
+    # P(len >= k) = 1/k for the Pareto (density x^-2) floor sampler
+    for k in (2, 10, 100):
+        assert abs((lens >= k).mean() - 1.0 / k) < 4 * np.sqrt(1.0 / k / lens.size) + 1e-3
+    for r in range(0, 20000, 97):
+        seg = ci[rp[r]:rp[r + 1]].astype(np.int64)
+        assert np.all(np.diff(seg) > 0)
+    assert np.all(np.abs(va) <= 1.0)
+
+
+def test_uniform_family_and_bandwidth_window():
+    spec = mfp.GenSpec("uniform", values="uniform", rows=5000, nnz_lo=4, nnz_hi=64, bandwidth=0.01, seed=2)
+    rp, ci, _ = mfp.generate(spec)
+    lens = np.diff(rp.astype(np.int64))
+    assert lens.min() >= 4 and lens.max() <= 64
+    for r in range(0, 5000, 37):
+        seg = ci[rp[r]:rp[r + 1]].astype(np.int64)
+        assert np.all(np.diff(seg) > 0)
+        assert seg.max() - seg.min() < max(64, int(0.01 * 5000)) + 1
+
+
+def test_vector_generator_range_and_grid():
+    x = mfp.generate_vector(0, 100000)
+    assert x.min() >= -1 and x.max() < 1
+    assert np.all((x.astype(np.float64) * 2 ** 23) == np.round(x.astype(np.float64) * 2 ** 23))
+
+
+# ---- measurement_class and compare_results vs the reference's sources ----------------------
+
+def test_measurement_class_semantics():
+    m = mfp.measurement_class("GPU CSR", 2e-3, 4e9, 1e9)
+    assert m.get_effective_bandwidth() == 4e9 / (2e-3 * 1e9)
+    acc = mfp.measurement_class()
+    for el in (1e-3, 3e-3):
+        acc += mfp.measurement_class("GPU ELL", el, 1e6, 2e6)
+    acc.finalize()
+    assert acc.get_format() == "GPU ELL"
+    assert acc.get_elapsed() == pytest.approx(2e-3, rel=1e-15)
+
+
+def test_measurement_class_matches_reference_build():
+    import oracle
+    ref = oracle.ref_library()
+    if ref is None:
+        pytest.skip("oracle/_ref not built")
+    el = np.array([1e-3, 2.5e-5, 7e-2])
+    by = np.array([1e9, 3e7, 5.5e10])
+    op = np.array([2e8, 1e6, 4e9])
+    out = np.zeros(3)
+    ref.ref_measurement_mean(3, el, by, op, out)
+    acc = mfp.measurement_class()
+    for i in range(3):
+        acc += mfp.measurement_class("x", el[i], by[i], op[i])
+    acc.finalize()
+    assert (acc.get_elapsed(), acc.get_effective_bandwidth(), acc.get_computational_throughput()) == tuple(out)
+
+
+def _capture_fd2(fn):
+    import tempfile
+    sys.stderr.flush()
+    saved = os.dup(2)
+    with tempfile.TemporaryFile() as tf:
+        os.dup2(tf.fileno(), 2)
+        try:
+            fn()
+        finally:
+            os.dup2(saved, 2)
+            os.close(saved)
+        tf.seek(0)
+        return tf.read().decode()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_compare_results_text_matches_reference_build(dtype):
+    import oracle
+    ref = oracle.ref_library()
+    if ref is None:
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(0)
+    b = rng.uniform(-1, 1, 1000).astype(dtype)
+    for a in (b.copy(), b + dtype(1e-3), np.where(np.arange(1000) == 417, b + 1, b).astype(dtype)):
+        a = np.ascontiguousarray(a, dtype)
+        buf = C.create_string_buffer(4096)
+        ref.ref_compare_results(0 if dtype == np.float32 else 1, 1000, a.ctypes.data_as(C.c_void_p),
+                                b.ctypes.data_as(C.c_void_p), buf, 4096)
+        mine = _capture_fd2(lambda: mfp.compare_results(1000, a, b))
+        assert mine == buf.value.decode()
+        # and the printed error equals the oracle's value
+        if mine:
+            assert float(mine.split()[1]) == pytest.approx(oracle.compare_error(a, b), rel=1e-5)
