@@ -63,6 +63,23 @@ def time_plan(mfp, plan, xd, yd, reps, flush, timer):
     return ms
 
 
+def measured_copy_gbs(mfp, timer, nbytes=1 << 30, reps=5):
+    """HBM copy rate of a 16-byte streaming copy (read + write bytes / time)."""
+    from matrix_format_performance_amd import _lib as L
+    src = mfp.DeviceArray(nbytes // 4, np.float32)
+    dst = mfp.DeviceArray(nbytes // 4, np.float32)
+    src.fill(1.0)
+    ms = []
+    for _ in range(reps + 1):
+        timer.start()
+        L.call("mfp_stream_copy", dst.ptr, src.ptr, nbytes, None)
+        timer.stop()
+        ms.append(timer.ms())
+    src.free()
+    dst.free()
+    return 2 * nbytes / (statistics.median(ms[1:]) * 1e-3) / 1e9
+
+
 def cpu_baseline(rp, ci, va, x, nbytes, threads):
     import oracle
     _, t1 = oracle.spmv_seq_timed(rp, ci, va, x, reps=3)
@@ -97,6 +114,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
+    ap.add_argument("--bandwidth", type=float, default=1.0,
+                    help="power-law only: column window as a fraction of cols (study knob)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -112,7 +131,12 @@ def main():
     info = mfp.device_info()
     log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}")
     t0 = time.time()
-    rp, ci, va = mfp.workload(args.workload, dtype)
+    if args.workload == "powerlaw_1m" and args.bandwidth < 1.0:
+        rp, ci, va = mfp.generate(mfp.GenSpec("powerlaw", values="uniform", rows=1 << 20, nnz_lo=1,
+                                              nnz_hi=50000, alpha=2.0, seed=0,
+                                              bandwidth=args.bandwidth), dtype)
+    else:
+        rp, ci, va = mfp.workload(args.workload, dtype)
     rows, nnz = rp.size - 1, ci.size
     cols = rows
     x = mfp.generate_vector(1, cols, dtype)
@@ -125,6 +149,8 @@ def main():
     yd = mfp.DeviceArray(rows, dtype)
     timer = mfp.Timer()
     flush = not args.no_flush
+    copy_gbs = measured_copy_gbs(mfp, timer)
+    log(f"[bench] measured 16-byte copy: {copy_gbs:.1f} GB/s")
 
     fmts = list(mfp.FORMATS) if args.format == "auto" else [args.format]
     table = {}
@@ -202,6 +228,8 @@ def main():
             "wall_s_timed_region_incl_flushes": round(t_wall, 4),
             "max_norm_err_vs_cpu_reference": err,
             "gflops": round(2 * nnz / (mean_ms * 1e-3) / 1e9, 2),
+            "measured_copy_GB_per_s": round(copy_gbs, 1),
+            "frac_of_measured_copy": round(value / copy_gbs, 4),
         },
         "pct_hbm_roofline": round(100 * value / HBM_PEAK_GBS, 2),
         "formats": table,

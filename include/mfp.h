@@ -74,6 +74,9 @@ int mfp_fill(int dtype, void *dst, uint64_t n, double value, mfp_stream s);
 /* Overwrite `bytes` of scratch so the 256 MB Infinity Cache holds no matrix
  * data before a timed run (pass >= 512 MB). */
 int mfp_flush_cache(uint64_t bytes, mfp_stream s);
+/* dst = src with 16-byte nontemporal loads: the measured-copy reference for
+ * HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters. */
+int mfp_stream_copy(void *dst, const void *src, uint64_t bytes, mfp_stream s);
 
 /* Event pair timing on stream s (hipEventElapsedTime, milliseconds). */
 typedef struct mfp_timer_s *mfp_timer;

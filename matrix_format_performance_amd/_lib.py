@@ -112,6 +112,7 @@ SIGNATURES = {
     "mfp_memcpy_d2d": (i32, [vp, vp, sz, vp]),
     "mfp_fill": (i32, [i32, vp, u64, dbl, vp]),
     "mfp_flush_cache": (i32, [u64, vp]),
+    "mfp_stream_copy": (i32, [vp, vp, u64, vp]),
     "mfp_timer_create": (i32, [P(vp)]),
     "mfp_timer_start": (i32, [vp, vp]),
     "mfp_timer_stop": (i32, [vp, vp]),

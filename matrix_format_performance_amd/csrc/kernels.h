@@ -18,8 +18,13 @@ struct adaptive_args {
   uint32_t nchunks = 0;
   uint32_t chunk = 0;
   void *partials = nullptr; // nchunks values
-  uint32_t *tickets = nullptr; // nchunks counters, zero between calls
 };
+
+// CSR-Adaptive defaults: a row block holds up to this many nonzeros, which a
+// 256-lane workgroup reads as two 16-byte vectors per lane.
+inline uint32_t default_nnz_per_wg(int dtype) { return dtype == 1 /*MFP_F64*/ ? 1020u : 2040u; }
+constexpr uint32_t DEFAULT_LONG_ROW = 4096, DEFAULT_CHUNK = 2048;
+uint32_t adaptive_vectors_per_lane(int dtype, uint32_t nnz_per_wg);
 
 int launch_csr_scalar(int dtype, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
                       const void *a, const void *x, void *y, hipStream_t s);
@@ -46,7 +51,6 @@ struct adaptive_layout {
   uint4 *chunks = nullptr;
   uint32_t nchunks = 0;
   void *partials = nullptr;
-  uint32_t *tickets = nullptr;
 };
 int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t nnz_per_wg,
                           uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s);

@@ -39,14 +39,13 @@ void free_adaptive_layout(adaptive_layout *l)
   dfree(l->blocks);
   dfree(l->chunks);
   dfree(l->partials);
-  dfree(l->tickets);
   *l = adaptive_layout{};
 }
 
 int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t W,
                           uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s)
 {
-  MFP_ARG(W >= 1 && W <= 16384, "nnz_per_wg must be in [1, 16384]");
+  MFP_ARG(W >= 1 && adaptive_vectors_per_lane(dtype, W) <= 16, "nnz_per_wg too large");
   MFP_ARG(chunk >= 256, "chunk must be >= 256");
   MFP_ARG(long_row >= chunk, "long_row must be >= chunk");
   adaptive_layout l;
@@ -61,11 +60,8 @@ int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t
   }
   if (st == MFP_OK) st = dmalloc((void **)&l.chunks, (uint64_t)l.nchunks * sizeof(uint4));
   if (st == MFP_OK) st = dmalloc(&l.partials, (uint64_t)l.nchunks * dtype_size(dtype));
-  if (st == MFP_OK) st = dmalloc((void **)&l.tickets, (uint64_t)l.nchunks * 4);
-  if (st == MFP_OK && l.nchunks) {
+  if (st == MFP_OK && l.nchunks)
     k_chunk_fill<<<ceil_div_u32(l.nblocks, 256), 256, 0, s>>>(l.blocks, l.nblocks, cnt, base, l.chunks);
-    st = fill_u32(l.tickets, 0, l.nchunks, s);
-  }
   if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
   dfree(cnt);
   dfree(base);
@@ -297,9 +293,9 @@ int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *t)
   std::memset(t, 0, sizeof(*t));
   const uint32_t avg = m->rows ? (uint32_t)((m->nnz + m->rows - 1) / m->rows) : 1;
   t->lanes = std::min<uint32_t>(64, std::max<uint32_t>(2, next_pow2_u32(std::max<uint32_t>(1, avg))));
-  t->nnz_per_wg = 1024;
-  t->chunk = 2048;
-  t->long_row = 4096;
+  t->nnz_per_wg = default_nnz_per_wg(m->dtype);
+  t->chunk = DEFAULT_CHUNK;
+  t->long_row = DEFAULT_LONG_ROW;
   t->ell_width = 0;
   t->hyb_percent = 0.2; // main.cpp:240
   t->hyb_width = 0;
@@ -345,7 +341,7 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
     st = build_adaptive_layout(m->dtype, m->rows, m->row_ptr, t.nnz_per_wg, t.long_row, t.chunk,
                                &p->ad, s);
     if (st != MFP_OK) return fail(st);
-    p->device_bytes = ((uint64_t)p->ad.nblocks + 1) * 4 + (uint64_t)p->ad.nchunks * (16 + vs + 4);
+    p->device_bytes = ((uint64_t)p->ad.nblocks + 1) * 4 + (uint64_t)p->ad.nchunks * (16 + vs);
     break;
   case MFP_ELL:
   case MFP_HYB:
@@ -446,7 +442,6 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
     g.nchunks = p->ad.nchunks;
     g.chunk = p->t.chunk;
     g.partials = p->ad.partials;
-    g.tickets = p->ad.tickets;
     return launch_csr_adaptive(dt, g, m->values, x, y, s);
   }
   case MFP_ELL:

@@ -230,7 +230,7 @@ int mfp_gpu_csr_adaptive_spmv(int dtype, const mfp_host_csr *A, mfp_buffer Ad, m
   MFP_CHECK(check_dtype(dtype));
   MFP_ARG(out, "out is null");
   MFP_CHECK(upload_csr(dtype, A, Ad, cd, rd, xd, yd, 0.0));
-  const uint32_t W = 1024, chunk = 2048, long_row = 4096;
+  const uint32_t W = default_nnz_per_wg(dtype), chunk = DEFAULT_CHUNK, long_row = DEFAULT_LONG_ROW;
   adaptive_layout lay;
   MFP_CHECK(build_adaptive_layout(dtype, A->rows, (const uint32_t *)mfp_buffer_ptr(rd), W,
                                   long_row, chunk, &lay, nullptr));
@@ -246,7 +246,6 @@ int mfp_gpu_csr_adaptive_spmv(int dtype, const mfp_host_csr *A, mfp_buffer Ad, m
   g.nchunks = lay.nchunks;
   g.chunk = chunk;
   g.partials = lay.partials;
-  g.tickets = lay.tickets;
   double sec = 0;
   int st = run_timed([&] {
     return launch_csr_adaptive(dtype, g, mfp_buffer_ptr(Ad), mfp_buffer_ptr(xd), mfp_buffer_ptr(yd), nullptr);

@@ -1,6 +1,11 @@
 // SpMV kernels for gfx950: CSR-scalar, CSR-vector, CSR-Adaptive, ELL, COO,
 // SCOO and the two hybrid ELL+COO forms.
 //
+// SpMV is bound by HBM bandwidth, so every kernel keeps several independent
+// loads in flight per lane (loads are issued for a group of elements before
+// any of them is used) and streams the matrix with nontemporal loads, which
+// leaves L2 / Infinity Cache to the gathered x entries.
+//
 // Summation orders (bit-exact against the oracle's emulators where the
 // kernel is deterministic):
 //   CSR-scalar  sequential fma per row            (cpu_matrix_multiplier.cpp:20-61)
@@ -14,12 +19,31 @@
 
 namespace mfp {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// 16-byte value vectors and the matching column-index vectors.
+template <typename T> struct vec16;
+template <> struct vec16<float> {
+  using V = f32x4;
+  using I = u32x4;
+  static constexpr uint32_t N = 4;
+};
+template <> struct vec16<double> {
+  using V = f64x2;
+  using I = u32x2;
+  static constexpr uint32_t N = 2;
+};
+
 // ---------------------------------------------------------------------------
 // CSR-scalar: one lane per row, as gpu_matrix_multiplier.cu:22-43, but the
 // wave stages its rows' contiguous nnz range through LDS with coalesced loads
-// so each lane then walks only its own row.  Order: sequential fma.
+// (all of a chunk's loads issued before use) so each lane then walks only its
+// own row.  Order: sequential fma.
 // ---------------------------------------------------------------------------
-constexpr uint32_t SCALAR_CHUNK = 512;
+constexpr uint32_t SCALAR_PER_LANE = 8, SCALAR_CHUNK = 64 * SCALAR_PER_LANE;
 
 template <typename T>
 __global__ __launch_bounds__(64) void k_csr_scalar(uint32_t rows, const uint32_t *__restrict__ rp,
@@ -40,9 +64,21 @@ __global__ __launch_bounds__(64) void k_csr_scalar(uint32_t rows, const uint32_t
   T dot = 0;
   for (uint32_t c0 = b; c0 < e; c0 += SCALAR_CHUNK) {
     const uint32_t n = min(SCALAR_CHUNK, e - c0);
-    for (uint32_t j = lane; j < n; j += 64u) {
-      va[j] = ldg_stream(a + c0 + j);
-      vx[j] = x[ldg_stream(ci + c0 + j)];
+    T av[SCALAR_PER_LANE];
+    uint32_t cv[SCALAR_PER_LANE];
+#pragma unroll
+    for (uint32_t k = 0; k < SCALAR_PER_LANE; k++) {
+      const uint32_t j = lane + 64u * k;
+      av[k] = j < n ? ldg_stream(a + c0 + j) : (T)0;
+      cv[k] = j < n ? ldg_stream(ci + c0 + j) : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < SCALAR_PER_LANE; k++) {
+      const uint32_t j = lane + 64u * k;
+      if (j < n) {
+        va[j] = av[k];
+        vx[j] = x[cv[k]];
+      }
     }
     __syncthreads();
     const uint32_t lo = max(rs, c0), hi = min(re, c0 + n);
@@ -55,7 +91,7 @@ __global__ __launch_bounds__(64) void k_csr_scalar(uint32_t rows, const uint32_t
 // ---------------------------------------------------------------------------
 // CSR-vector: G lanes per row (G = 2..64; the reference uses a 32-lane warp,
 // gpu_matrix_multiplier.cu:331-362).  Lane l accumulates elements l, l+G, ...
-// with fma, then group_reduce<G> (halving tree).
+// with fma (four elements' loads issued together), then group_reduce<G>.
 // ---------------------------------------------------------------------------
 template <typename T, int G>
 __global__ __launch_bounds__(256) void k_csr_vector(uint32_t rows, const uint32_t *__restrict__ rp,
@@ -69,7 +105,19 @@ __global__ __launch_bounds__(256) void k_csr_vector(uint32_t rows, const uint32_
   T dot = 0;
   if (row < rows) {
     const uint32_t b = rp[row], e = rp[row + 1];
-    for (uint32_t i = b + lane; i < e; i += G) dot = fma_t(ldg_stream(a + i), x[ldg_stream(ci + i)], dot);
+    uint32_t i = b + lane;
+    for (; i + 3u * G < e; i += 4u * G) {
+      const T a0 = ldg_stream(a + i), a1 = ldg_stream(a + i + G);
+      const T a2 = ldg_stream(a + i + 2 * G), a3 = ldg_stream(a + i + 3 * G);
+      const uint32_t c0 = ldg_stream(ci + i), c1 = ldg_stream(ci + i + G);
+      const uint32_t c2 = ldg_stream(ci + i + 2 * G), c3 = ldg_stream(ci + i + 3 * G);
+      const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+      dot = fma_t(a0, x0, dot);
+      dot = fma_t(a1, x1, dot);
+      dot = fma_t(a2, x2, dot);
+      dot = fma_t(a3, x3, dot);
+    }
+    for (; i < e; i += G) dot = fma_t(ldg_stream(a + i), x[ldg_stream(ci + i)], dot);
   }
   dot = group_reduce<G>(dot);
   if (lane == 0 && row < rows) y[row] = dot;
@@ -78,15 +126,21 @@ __global__ __launch_bounds__(256) void k_csr_vector(uint32_t rows, const uint32_
 // ---------------------------------------------------------------------------
 // CSR-Adaptive (csr_adaptive_spmv.cu:26-178 redesigned for 256-lane
 // workgroups and LDS-sized row blocks).
-//   multi-row block  (CSR-Stream): rounded products a*x staged in LDS (up to
-//     nnz_per_wg of them), then each row summed by t lanes strided by t
-//     (t = min(64, prev_pow2(256 / rows))) and a halving tree; t == 1 sums
-//     each row sequentially.
+//   multi-row block  (CSR-Stream): the block's nnz range is read with 16-byte
+//     vector loads (K vectors per lane, all in flight together), rounded
+//     products a*x staged in LDS (up to nnz_per_wg of them).  The reduction
+//     is balanced whatever the row lengths: lane i sums the products
+//     [i*E, i*E + E) (E = ceil(nnz_per_wg / 256)) left to right, restarting
+//     at every row boundary; rows inside one lane finish there, and the
+//     partial a lane carries into the next enters a 256-lane Hillis-Steele
+//     segmented scan; a row ending in lane j that began earlier is
+//     scan[j-1] + lane j's partial.
 //   single-row block (CSR-Vector/VectorL): 256 lanes strided with fma, a
 //     64-lane halving tree per wave, then a halving tree over the 4 waves.
 //   rows longer than long_row: split into `chunk`-element pieces, one
-//     workgroup each; the last piece to finish adds the partials left to
-//     right (ticket counter; it resets the counter for the next call).
+//     workgroup each, reduced like a single-row block; k_adaptive_finish
+//     (launched next on the same stream) adds a row's pieces left to right.
+//     The pieces are dispatched first, round-robin over the XCDs.
 // ---------------------------------------------------------------------------
 constexpr uint32_t AD_T = 256;
 
@@ -95,7 +149,19 @@ __device__ __forceinline__ T wg_row_sum(const uint32_t *__restrict__ ci, const T
                                         const T *__restrict__ x, uint32_t b, uint32_t e, T *wsum)
 {
   T d = 0;
-  for (uint32_t i = b + threadIdx.x; i < e; i += AD_T) d = fma_t(ldg_stream(a + i), x[ldg_stream(ci + i)], d);
+  uint32_t i = b + threadIdx.x;
+  for (; i + 3 * AD_T < e; i += 4 * AD_T) {
+    const T a0 = ldg_stream(a + i), a1 = ldg_stream(a + i + AD_T);
+    const T a2 = ldg_stream(a + i + 2 * AD_T), a3 = ldg_stream(a + i + 3 * AD_T);
+    const uint32_t c0 = ldg_stream(ci + i), c1 = ldg_stream(ci + i + AD_T);
+    const uint32_t c2 = ldg_stream(ci + i + 2 * AD_T), c3 = ldg_stream(ci + i + 3 * AD_T);
+    const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    d = fma_t(a0, x0, d);
+    d = fma_t(a1, x1, d);
+    d = fma_t(a2, x2, d);
+    d = fma_t(a3, x3, d);
+  }
+  for (; i < e; i += AD_T) d = fma_t(ldg_stream(a + i), x[ldg_stream(ci + i)], d);
   d = group_reduce<64>(d);
   if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = d;
   __syncthreads();
@@ -108,43 +174,36 @@ __device__ __forceinline__ T wg_row_sum(const uint32_t *__restrict__ ci, const T
   return r;
 }
 
-template <typename T>
+template <typename T, uint32_t K>
 __global__ __launch_bounds__(AD_T) void k_csr_adaptive(
     uint32_t rows, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
     const T *__restrict__ a, const T *__restrict__ x, T *__restrict__ y,
-    const uint32_t *__restrict__ blocks, uint32_t nblocks, uint32_t long_row,
-    const uint4 *__restrict__ chunks, uint32_t chunk, T *__restrict__ partials,
-    uint32_t *__restrict__ tickets)
+    const uint32_t *__restrict__ blocks, uint32_t nblocks, uint32_t nnz_per_wg, uint32_t long_row,
+    const uint4 *__restrict__ chunks, uint32_t nchunks, uint32_t nchunks_pad, uint32_t chunk,
+    T *__restrict__ partials)
 {
+  using VT = vec16<T>;
+  constexpr uint32_t N = VT::N;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *cache = reinterpret_cast<T *>(smem_raw);
+  uint32_t *rs = reinterpret_cast<uint32_t *>(smem_raw + ((nnz_per_wg * sizeof(T) + 15u) & ~15u));
   __shared__ T wsum[4];
-  const uint32_t item = xcd_block(blockIdx.x, gridDim.x);
+  __shared__ T sv[AD_T];
+  __shared__ uint32_t sf[AD_T];
   const uint32_t tid = threadIdx.x;
 
-  if (item >= nblocks) {
-    const uint4 c4 = chunks[item - nblocks]; // {row, piece, first item, pieces}
+  if (blockIdx.x < nchunks_pad) {
+    if (blockIdx.x >= nchunks) return;
+    const uint4 c4 = chunks[blockIdx.x]; // {row, piece, first item, pieces}
     const uint32_t s = rp[c4.x], e = rp[c4.x + 1];
     const uint32_t cb = s + c4.y * chunk;
     const uint32_t ce = min(e, cb + chunk);
     const T p = wg_row_sum(ci, a, x, cb, ce, wsum);
-    if (tid == 0) {
-      partials[c4.z + c4.y] = p;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      const uint32_t old = __hip_atomic_fetch_add(tickets + c4.z, 1u, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-      if (old == c4.w - 1u) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        T acc = __hip_atomic_load(partials + c4.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (uint32_t i = 1; i < c4.w; i++)
-          acc = acc + __hip_atomic_load(partials + c4.z + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        y[c4.x] = acc;
-        __hip_atomic_store(tickets + c4.z, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
+    if (tid == 0) partials[c4.z + c4.y] = p;
     return;
   }
 
+  const uint32_t item = xcd_block(blockIdx.x - nchunks_pad, gridDim.x - nchunks_pad);
   const uint32_t r0 = blocks[item], r1 = blocks[item + 1];
   if (r1 <= r0) return;
   if (r1 - r0 == 1) {
@@ -155,35 +214,134 @@ __global__ __launch_bounds__(AD_T) void k_csr_adaptive(
     return;
   }
 
-  // CSR-Stream
+  // CSR-Stream: vectors [v0, v1) cover the block's elements [base, base + n)
   const uint32_t base = rp[r0];
   const uint32_t n = rp[r1] - base;
-  for (uint32_t i = tid; i < n; i += AD_T) cache[i] = ldg_stream(a + base + i) * x[ldg_stream(ci + base + i)];
-  __syncthreads();
+  const uint32_t v0 = base / N, v1 = (base + n + N - 1) / N;
+  const typename VT::V *av = reinterpret_cast<const typename VT::V *>(a);
+  const typename VT::I *cv = reinterpret_cast<const typename VT::I *>(ci);
+  typename VT::V va[K];
+  typename VT::I vc[K];
+#pragma unroll
+  for (uint32_t k = 0; k < K; k++) {
+    const uint32_t v = v0 + tid + k * AD_T;
+    if (v < v1) {
+      va[k] = ldg_stream(av + v);
+      vc[k] = ldg_stream(cv + v);
+    }
+  }
+  T xv[K][N];
+#pragma unroll
+  for (uint32_t k = 0; k < K; k++) {
+    const uint32_t v = v0 + tid + k * AD_T;
+#pragma unroll
+    for (uint32_t j = 0; j < N; j++) {
+      const uint32_t e = v * N + j;
+      xv[k][j] = (v < v1 && e >= base && e < base + n) ? x[vc[k][j]] : (T)0;
+    }
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < K; k++) {
+    const uint32_t v = v0 + tid + k * AD_T;
+#pragma unroll
+    for (uint32_t j = 0; j < N; j++) {
+      const uint32_t e = v * N + j;
+      if (v < v1 && e >= base && e < base + n) cache[e - base] = va[k][j] * xv[k][j];
+    }
+  }
+  // block-relative row starts rs[0..nr]
   const uint32_t nr = r1 - r0;
-  const uint32_t q = AD_T / nr;
-  uint32_t t = 1;
-  if (q >= 2) {
-    t = 1u << (31 - __clz(q));
-    if (t > 64) t = 64;
-  }
-  if (t == 1) {
-    for (uint32_t lr = tid; lr < nr; lr += AD_T) {
-      const uint32_t f = rp[r0 + lr] - base, l = rp[r0 + lr + 1] - base;
-      T d = 0;
-      for (uint32_t j = f; j < l; j++) d = d + cache[j];
-      y[r0 + lr] = d;
+  for (uint32_t lr = tid; lr <= nr; lr += AD_T) rs[lr] = rp[r0 + lr] - base;
+  __syncthreads();
+  for (uint32_t lr = tid; lr < nr; lr += AD_T)
+    if (rs[lr] == rs[lr + 1]) y[r0 + lr] = (T)0; // empty rows
+
+  // lane's run [e0, e1): sums split at row boundaries
+  const uint32_t E = (nnz_per_wg + AD_T - 1) / AD_T;
+  const uint32_t e0 = tid * E;
+  const uint32_t e1 = min(n, e0 + E);
+  T carry_v = 0;       // partial carried into the next lane
+  uint32_t carry_f = 1; // 1: no chain comes in from the left
+  uint32_t frow = 0xFFFFFFFFu;
+  T fval = 0;
+  if (e0 < e1) {
+    uint32_t lo = 0, hi = nr; // rs[lo] <= e0 < rs[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (rs[mid] <= e0)
+        lo = mid;
+      else
+        hi = mid;
     }
-  } else {
-    const uint32_t lr = tid / t, k = tid % t;
+    uint32_t cur = lo;
+    uint32_t cur_start = rs[cur], cur_end = rs[cur + 1];
+    uint32_t seg = e0;
     T d = 0;
-    if (lr < nr) {
-      const uint32_t f = rp[r0 + lr] - base, l = rp[r0 + lr + 1] - base;
-      for (uint32_t j = f + k; j < l; j += t) d = d + cache[j];
+    for (uint32_t e = e0; e < e1; e++) {
+      while (cur_end <= e) {
+        if (seg < e) {
+          if (seg == cur_start) {
+            y[r0 + cur] = d;
+          } else {
+            frow = cur;
+            fval = d;
+          }
+        }
+        d = 0;
+        seg = e;
+        cur++;
+        cur_start = cur_end;
+        cur_end = rs[cur + 1];
+      }
+      d = d + cache[e];
     }
-    d = group_reduce_rt(d, t);
-    if (k == 0 && lr < nr) y[r0 + lr] = d;
+    const bool starts = seg == cur_start;
+    if (e1 == cur_end) {
+      if (starts) {
+        y[r0 + cur] = d;
+      } else {
+        frow = cur;
+        fval = d;
+      }
+    } else {
+      carry_v = d;
+      carry_f = starts ? 1u : 0u;
+    }
   }
+  // segmented inclusive scan of the carried partials (Hillis-Steele)
+  sv[tid] = carry_v;
+  sf[tid] = carry_f;
+  __syncthreads();
+  for (uint32_t off = 1; off < AD_T; off <<= 1) {
+    const bool take = sf[tid] == 0u && tid >= off;
+    T pv = 0;
+    uint32_t pf = 0;
+    if (take) {
+      pv = sv[tid - off];
+      pf = sf[tid - off];
+    }
+    __syncthreads();
+    if (take) {
+      sv[tid] = pv + sv[tid];
+      sf[tid] = pf;
+    }
+    __syncthreads();
+  }
+  if (frow != 0xFFFFFFFFu) y[r0 + frow] = sv[tid - 1] + fval;
+}
+
+// One lane per long row: add its pieces left to right.
+template <typename T>
+__global__ void k_adaptive_finish(const uint4 *__restrict__ chunks, uint32_t nchunks,
+                                  const T *__restrict__ partials, T *__restrict__ y)
+{
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nchunks) return;
+  const uint4 c4 = chunks[j];
+  if (c4.y != 0) return;
+  T acc = partials[c4.z];
+  for (uint32_t i = 1; i < c4.w; i++) acc = acc + partials[c4.z + i];
+  y[c4.x] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -205,10 +363,11 @@ __device__ __forceinline__ void ell_row(uint32_t r, uint32_t rows, uint32_t w,
     const T a2 = ldg_stream(ev + o + 2ull * rows), a3 = ldg_stream(ev + o + 3ull * rows);
     const uint32_t c0 = ldg_stream(ec + o), c1 = ldg_stream(ec + o + rows);
     const uint32_t c2 = ldg_stream(ec + o + 2ull * rows), c3 = ldg_stream(ec + o + 3ull * rows);
-    d = fma_t(a0, x[c0], d);
-    d = fma_t(a1, x[c1], d);
-    d = fma_t(a2, x[c2], d);
-    d = fma_t(a3, x[c3], d);
+    const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    d = fma_t(a0, x0, d);
+    d = fma_t(a1, x1, d);
+    d = fma_t(a2, x2, d);
+    d = fma_t(a3, x3, d);
     o += 4ull * rows;
   }
   for (; k < w; k++, o += rows) d = fma_t(ldg_stream(ev + o), x[ldg_stream(ec + o)], d);
@@ -228,20 +387,17 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t rows, uint32_t w, const ui
 }
 
 // ---------------------------------------------------------------------------
-// COO (gpu_matrix_multiplier.cu:622-638): a wave takes 64 consecutive
-// entries (row-sorted); a segmented suffix sum over equal row keys leaves
-// each run's total in its first lane, which issues one atomic add.
+// COO (gpu_matrix_multiplier.cu:622-638): a wave takes 4 x 64 consecutive
+// (row-sorted) entries, all loads in flight together.  Per 64-entry tile a
+// segmented suffix sum over equal row keys leaves each run's total in its
+// first lane, which issues one atomic add.
 // ---------------------------------------------------------------------------
+constexpr uint32_t COO_TILES = 4;
+
 template <typename T>
-__device__ __forceinline__ void coo_tile(uint64_t base, uint64_t n, const uint32_t *__restrict__ cr,
-                                         const uint32_t *__restrict__ cc, const T *__restrict__ cv,
-                                         const T *__restrict__ x, T *__restrict__ y)
+__device__ __forceinline__ void coo_segment_add(uint32_t r, T p, bool valid, T *__restrict__ y)
 {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t i = base + lane;
-  const bool valid = i < n;
-  const uint32_t r = valid ? ldg_stream(cr + i) : 0xFFFFFFFFu;
-  T p = valid ? ldg_stream(cv + i) * x[ldg_stream(cc + i)] : (T)0;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const uint32_t r2 = __shfl_down(r, off, 64);
@@ -253,14 +409,38 @@ __device__ __forceinline__ void coo_tile(uint64_t base, uint64_t n, const uint32
 }
 
 template <typename T>
+__device__ __forceinline__ void coo_wave(uint64_t base, uint64_t n, const uint32_t *__restrict__ cr,
+                                         const uint32_t *__restrict__ cc, const T *__restrict__ cv,
+                                         const T *__restrict__ x, T *__restrict__ y)
+{
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t r[COO_TILES], c[COO_TILES];
+  T v[COO_TILES], xv[COO_TILES];
+#pragma unroll
+  for (uint32_t k = 0; k < COO_TILES; k++) {
+    const uint64_t i = base + 64u * k + lane;
+    const bool ok = i < n;
+    r[k] = ok ? ldg_stream(cr + i) : 0xFFFFFFFFu;
+    c[k] = ok ? ldg_stream(cc + i) : 0u;
+    v[k] = ok ? ldg_stream(cv + i) : (T)0;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < COO_TILES; k++) xv[k] = (base + 64u * k + lane < n) ? x[c[k]] : (T)0;
+#pragma unroll
+  for (uint32_t k = 0; k < COO_TILES; k++)
+    coo_segment_add(r[k], v[k] * xv[k], base + 64u * k + lane < n, y);
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_coo(uint64_t n, const uint32_t *__restrict__ cr,
                                              const uint32_t *__restrict__ cc,
                                              const T *__restrict__ cv, const T *__restrict__ x,
                                              T *__restrict__ y)
 {
+  const uint64_t span = 64u * COO_TILES;
   const uint64_t waves = (uint64_t)gridDim.x * 4u;
   uint64_t w = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
-  for (; w * 64u < n; w += waves) coo_tile(w * 64u, n, cr, cc, cv, x, y);
+  for (; w * span < n; w += waves) coo_wave(w * span, n, cr, cc, cv, x, y);
 }
 
 // Fused hybrid (gpu_matrix_multiplier.cu:969-1002): workgroups [0, ell_blocks)
@@ -281,9 +461,10 @@ __global__ __launch_bounds__(256) void k_hyb_atomic(uint32_t rows, uint32_t w,
     if (r < rows) ell_row<T, true>(r, rows, w, ec, ev, x, y);
     return;
   }
+  const uint64_t span = 64u * COO_TILES;
   const uint64_t waves = (uint64_t)(gridDim.x - ell_blocks) * 4u;
   uint64_t wv = (uint64_t)(b - ell_blocks) * 4u + (threadIdx.x >> 6);
-  for (; wv * 64u < ncoo; wv += waves) coo_tile(wv * 64u, ncoo, cr, cc, cv, x, y);
+  for (; wv * span < ncoo; wv += waves) coo_wave(wv * span, ncoo, cr, cc, cv, x, y);
 }
 
 // ---------------------------------------------------------------------------
@@ -307,7 +488,23 @@ __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *_
   __syncthreads();
   const uint32_t b = sp[s], e = sp[s + 1];
   const uint32_t lane = tid & (L - 1u);
-  for (uint32_t i = b + tid; i < e; i += bt) {
+  uint32_t i = b + tid;
+  for (; i + 3 * bt < e; i += 4 * bt) {
+    uint32_t lr[4], c[4];
+    T v[4], xv[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      lr[k] = ldg_stream(cr + i + k * bt) - r0;
+      c[k] = ldg_stream(cc + i + k * bt);
+      v[k] = ldg_stream(cv + i + k * bt);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) xv[k] = x[c[k]];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++)
+      if (lr[k] < lim) atomicAdd(acc + lr[k] * L + lane, v[k] * xv[k]);
+  }
+  for (; i < e; i += bt) {
     const uint32_t lr = ldg_stream(cr + i) - r0;
     const T p = ldg_stream(cv + i) * x[ldg_stream(cc + i)];
     if (lr < lim) atomicAdd(acc + lr * L + lane, p);
@@ -363,25 +560,73 @@ int launch_csr_vector(int dtype, uint32_t lanes, uint32_t rows, const uint32_t *
   return csr_vector_t<double>(lanes, rows, rp, ci, (const double *)a, (const double *)x, (double *)y, s);
 }
 
+// Vectors a lane may need for a block of up to W elements at any alignment.
+uint32_t adaptive_vectors_per_lane(int dtype, uint32_t W)
+{
+  const uint32_t N = dtype == MFP_F32 ? 4 : 2;
+  const uint32_t maxvec = (W + 2 * N - 2) / N;
+  uint32_t k = (maxvec + AD_T - 1) / AD_T;
+  uint32_t p = 1;
+  while (p < k) p <<= 1;
+  return p;
+}
+
+template <typename T, uint32_t K>
+static void adaptive_launch_k(const adaptive_args &g, const T *a, const T *x, T *y, hipStream_t s,
+                              size_t lds, uint32_t grid, uint32_t nchunks_pad)
+{
+  k_csr_adaptive<T, K><<<grid, AD_T, lds, s>>>(g.rows, g.rp, g.ci, a, x, y, g.blocks, g.nblocks,
+                                               g.nnz_per_wg, g.long_row, g.chunks, g.nchunks,
+                                               nchunks_pad, g.chunk, (T *)g.partials);
+}
+
+// Dynamic LDS of k_csr_adaptive: the staged products, then the row starts.
+size_t adaptive_lds_bytes(int dtype, uint32_t W)
+{
+  return ((W * dtype_size(dtype) + 15u) & ~size_t(15)) + ((size_t)W + 2) * 4;
+}
+
+template <typename T>
+static int adaptive_t(const adaptive_args &g, const T *a, const T *x, T *y, hipStream_t s)
+{
+  // chunk items first; padded to a multiple of 8 so block ids keep their XCD
+  const uint32_t nchunks_pad = (g.nchunks + 7u) & ~7u;
+  const uint32_t grid = g.nblocks ? g.nblocks + nchunks_pad : nchunks_pad;
+  const size_t lds = adaptive_lds_bytes(sizeof(T) == 4 ? MFP_F32 : MFP_F64, g.nnz_per_wg);
+  const uint32_t K = adaptive_vectors_per_lane(sizeof(T) == 4 ? MFP_F32 : MFP_F64, g.nnz_per_wg);
+  const void *fn = nullptr;
+  switch (K) {
+  case 1: fn = (const void *)k_csr_adaptive<T, 1>; break;
+  case 2: fn = (const void *)k_csr_adaptive<T, 2>; break;
+  case 4: fn = (const void *)k_csr_adaptive<T, 4>; break;
+  case 8: fn = (const void *)k_csr_adaptive<T, 8>; break;
+  case 16: fn = (const void *)k_csr_adaptive<T, 16>; break;
+  default: set_error("nnz_per_wg too large"); return MFP_ERR_ARG;
+  }
+  if (lds > 65536) MFP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (grid) {
+    switch (K) {
+    case 1: adaptive_launch_k<T, 1>(g, a, x, y, s, lds, grid, nchunks_pad); break;
+    case 2: adaptive_launch_k<T, 2>(g, a, x, y, s, lds, grid, nchunks_pad); break;
+    case 4: adaptive_launch_k<T, 4>(g, a, x, y, s, lds, grid, nchunks_pad); break;
+    case 8: adaptive_launch_k<T, 8>(g, a, x, y, s, lds, grid, nchunks_pad); break;
+    default: adaptive_launch_k<T, 16>(g, a, x, y, s, lds, grid, nchunks_pad); break;
+    }
+    MFP_CHECK(launched());
+  }
+  if (g.nchunks) {
+    k_adaptive_finish<T><<<ceil_div_u32(g.nchunks, 256), 256, 0, s>>>(g.chunks, g.nchunks,
+                                                                       (const T *)g.partials, y);
+    MFP_CHECK(launched());
+  }
+  return MFP_OK;
+}
+
 int launch_csr_adaptive(int dtype, const adaptive_args &g, const void *a, const void *x, void *y,
                         hipStream_t s)
 {
-  const uint32_t grid = g.nblocks + g.nchunks;
-  if (grid == 0) return MFP_OK;
-  const size_t lds = (size_t)g.nnz_per_wg * dtype_size(dtype);
-  if (lds > 65536) {
-    const void *fn = dtype == MFP_F32 ? (const void *)k_csr_adaptive<float> : (const void *)k_csr_adaptive<double>;
-    MFP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  }
-  if (dtype == MFP_F32)
-    k_csr_adaptive<float><<<grid, AD_T, lds, s>>>(g.rows, g.rp, g.ci, (const float *)a, (const float *)x,
-                                                  (float *)y, g.blocks, g.nblocks, g.long_row,
-                                                  g.chunks, g.chunk, (float *)g.partials, g.tickets);
-  else
-    k_csr_adaptive<double><<<grid, AD_T, lds, s>>>(g.rows, g.rp, g.ci, (const double *)a, (const double *)x,
-                                                   (double *)y, g.blocks, g.nblocks, g.long_row,
-                                                   g.chunks, g.chunk, (double *)g.partials, g.tickets);
-  return launched();
+  if (dtype == MFP_F32) return adaptive_t<float>(g, (const float *)a, (const float *)x, (float *)y, s);
+  return adaptive_t<double>(g, (const double *)a, (const double *)x, (double *)y, s);
 }
 
 int launch_ell(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const void *ev,
@@ -398,7 +643,8 @@ int launch_ell(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const v
 
 static uint32_t coo_grid(uint64_t n)
 {
-  const uint64_t tiles = (n + 255) / 256; // 4 waves x 64 entries per block pass
+  const uint64_t per_block = 4u * 64u * COO_TILES; // 4 waves x COO_TILES tiles per pass
+  const uint64_t tiles = (n + per_block - 1) / per_block;
   const uint64_t cap = (uint64_t)std::max(1, props().cu_count) * 16u;
   return (uint32_t)std::max<uint64_t>(1, std::min(tiles, cap));
 }

@@ -38,6 +38,11 @@ LABELS = {
 
 _DT = {np.dtype(np.float32): L.MFP_F32, np.dtype(np.float64): L.MFP_F64}
 
+# CSR-Adaptive defaults of libmfp (kernels.h): row-block size per value type,
+# long-row threshold and split chunk.
+DEFAULT_NNZ_PER_WG = {np.dtype(np.float32): 2040, np.dtype(np.float64): 1020}
+DEFAULT_LONG_ROW, DEFAULT_CHUNK = 4096, 2048
+
 
 def dtype_code(dtype) -> int:
     d = np.dtype(dtype)

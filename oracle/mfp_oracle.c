@@ -471,6 +471,90 @@ void orc_spmv_ell_seq(int dtype, uint32_t rows, uint32_t width,
 DEFINE_WG_ROW(float, fmaf, f32)
 DEFINE_WG_ROW(double, fma, f64)
 
+/* Segmented reduction of one multi-row block (CSR-Stream).  Lane i of B
+ * sums the rounded products cache[i*E, i*E+E) (E = ceil(W/B)) left to right,
+ * restarting at every row boundary.  A row that starts and ends inside one
+ * lane is finished there.  The partial a lane carries into the next lane
+ * (its last row continues) enters a Hillis-Steele segmented inclusive scan
+ * over the B lanes: for off = 1, 2, 4, ...: a lane whose chain does not start
+ * in it adds the value `off` lanes to its left (s = s[i-off] + s) and takes
+ * that lane's flag.  A row ending in lane j that began earlier gets
+ * scan[j-1] + (lane j's partial). */
+#define DEFINE_STREAM(T, SUF)                                                 \
+  static void stream_block_##SUF(uint32_t r0, uint32_t r1, const uint32_t *rp, \
+                                 const T *cache, uint32_t W, uint32_t B,      \
+                                 T *y, T *sv, T *sv2, unsigned char *sf,      \
+                                 unsigned char *sf2, uint32_t *frow,          \
+                                 T *fval)                                     \
+  {                                                                           \
+    const uint32_t base = rp[r0], n = rp[r1] - base, nr = r1 - r0;            \
+    const uint32_t E = (W + B - 1) / B;                                       \
+    for (uint32_t lr = 0; lr < nr; lr++)                                      \
+      if (rp[r0 + lr] == rp[r0 + lr + 1])                                     \
+        y[r0 + lr] = 0;                                                       \
+    for (uint32_t i = 0; i < B; i++) {                                        \
+      sv[i] = 0;                                                              \
+      sf[i] = 1;                                                              \
+      frow[i] = 0xFFFFFFFFu;                                                  \
+      const uint32_t e0 = i * E;                                              \
+      const uint32_t e1 = e0 + E < n ? e0 + E : n;                            \
+      if (e0 >= e1)                                                           \
+        continue;                                                             \
+      uint32_t cur = 0;                                                       \
+      while (rp[r0 + cur + 1] - base <= e0)                                   \
+        cur++;                                                                \
+      T d = 0;                                                                \
+      uint32_t seg = e0;                                                      \
+      for (uint32_t e = e0; e < e1; e++) {                                    \
+        while (rp[r0 + cur + 1] - base <= e) {                                \
+          if (seg < e) {                                                      \
+            if (seg == rp[r0 + cur] - base)                                   \
+              y[r0 + cur] = d;                                                \
+            else {                                                            \
+              frow[i] = cur;                                                  \
+              fval[i] = d;                                                    \
+            }                                                                 \
+          }                                                                   \
+          d = 0;                                                              \
+          seg = e;                                                            \
+          cur++;                                                              \
+        }                                                                     \
+        d = d + cache[e];                                                     \
+      }                                                                       \
+      const int starts = seg == rp[r0 + cur] - base;                          \
+      const int ends = e1 == rp[r0 + cur + 1] - base;                         \
+      if (ends) {                                                             \
+        if (starts)                                                           \
+          y[r0 + cur] = d;                                                    \
+        else {                                                                \
+          frow[i] = cur;                                                      \
+          fval[i] = d;                                                        \
+        }                                                                     \
+      } else {                                                                \
+        sv[i] = d;                                                            \
+        sf[i] = (unsigned char)starts;                                        \
+      }                                                                       \
+    }                                                                         \
+    for (uint32_t off = 1; off < B; off <<= 1) {                              \
+      for (uint32_t i = 0; i < B; i++) {                                      \
+        sv2[i] = sv[i];                                                       \
+        sf2[i] = sf[i];                                                       \
+        if (!sf[i] && i >= off) {                                             \
+          sv2[i] = sv[i - off] + sv[i];                                       \
+          sf2[i] = sf[i - off];                                               \
+        }                                                                     \
+      }                                                                       \
+      memcpy(sv, sv2, sizeof(T) * B);                                         \
+      memcpy(sf, sf2, B);                                                     \
+    }                                                                         \
+    for (uint32_t i = 1; i < B; i++)                                          \
+      if (frow[i] != 0xFFFFFFFFu)                                             \
+        y[r0 + frow[i]] = sv[i - 1] + fval[i];                                \
+    (void)nr;                                                                 \
+  }
+DEFINE_STREAM(float, f32)
+DEFINE_STREAM(double, f64)
+
 #define DEFINE_ADAPTIVE(T, SUF)                                               \
   static void adaptive_##SUF(uint32_t rows, const uint32_t *rp,              \
                              const uint32_t *ci, const T *a, const T *x,      \
@@ -481,7 +565,10 @@ DEFINE_WG_ROW(double, fma, f64)
     uint32_t *blk = malloc(sizeof(uint32_t) * (nb + 1));                      \
     orc_fill_row_blocks(1, rows, rp, blk, W);                                 \
     T *cache = malloc(sizeof(T) * (W + 1));                                   \
-    T part[64];                                                               \
+    T *sv = malloc(sizeof(T) * B), *sv2 = malloc(sizeof(T) * B);              \
+    T *fval = malloc(sizeof(T) * B);                                          \
+    unsigned char *sf = malloc(B), *sf2 = malloc(B);                          \
+    uint32_t *frow = malloc(sizeof(uint32_t) * B);                            \
     for (uint32_t b = 0; b < nb; b++) {                                       \
       uint32_t r0 = blk[b], r1 = blk[b + 1];                                  \
       if (r1 <= r0)                                                           \
@@ -490,31 +577,8 @@ DEFINE_WG_ROW(double, fma, f64)
         uint32_t base = rp[r0], n = rp[r1] - base;                            \
         for (uint32_t i = 0; i < n; i++)                                      \
           cache[i] = a[base + i] * x[ci[base + i]];                           \
-        uint32_t nr = r1 - r0;                                                \
-        uint32_t q = B / nr;                                                  \
-        uint32_t t = q >= 2 ? orc_prev_pow_2(q) : 1;                          \
-        if (t > 64)                                                           \
-          t = 64;                                                             \
-        for (uint32_t r = r0; r < r1; r++) {                                  \
-          uint32_t f = rp[r] - base, l = rp[r + 1] - base;                    \
-          if (t == 1) {                                                       \
-            T d = 0;                                                          \
-            for (uint32_t j = f; j < l; j++)                                  \
-              d = d + cache[j];                                               \
-            y[r] = d;                                                         \
-          } else {                                                            \
-            for (uint32_t k = 0; k < t; k++) {                                \
-              T d = 0;                                                        \
-              for (uint32_t j = f + k; j < l; j += t)                         \
-                d = d + cache[j];                                             \
-              part[k] = d;                                                    \
-            }                                                                 \
-            for (uint32_t off = t / 2; off > 0; off /= 2)                     \
-              for (uint32_t k = 0; k < off; k++)                              \
-                part[k] = part[k] + part[k + off];                            \
-            y[r] = part[0];                                                   \
-          }                                                                   \
-        }                                                                     \
+        stream_block_##SUF(r0, r1, rp, cache, W, B, y, sv, sv2, sf, sf2,      \
+                           frow, fval);                                       \
       } else {                                                                \
         uint32_t s = rp[r0], e = rp[r0 + 1];                                  \
         if (e - s > long_row) {                                               \
@@ -534,6 +598,12 @@ DEFINE_WG_ROW(double, fma, f64)
     }                                                                         \
     free(cache);                                                              \
     free(blk);                                                                \
+    free(sv);                                                                 \
+    free(sv2);                                                                \
+    free(fval);                                                               \
+    free(sf);                                                                 \
+    free(sf2);                                                                \
+    free(frow);                                                               \
   }
 DEFINE_ADAPTIVE(float, f32)
 DEFINE_ADAPTIVE(double, f64)

@@ -54,7 +54,9 @@ def test_main_cpp_sequence(gpu, dtype):
         lanes *= 2
     check(m, "GPU CSR (vector)", exact=oracle.spmv_lanes(csr.row_ptr, csr.columns, csr.data, ones, lanes))
     m = mfp.gpu_csr_adaptive_spmv(csr, A, col_ids, row_ptr, x_gpu, y, x, reference_answer)
-    check(m, "GPU CSR-Adaptive", exact=oracle.spmv_adaptive(csr.row_ptr, csr.columns, csr.data, ones, 1024))
+    W = mfp.engine.DEFAULT_NNZ_PER_WG[np.dtype(dtype)]
+    check(m, "GPU CSR-Adaptive", exact=oracle.spmv_adaptive(csr.row_ptr, csr.columns, csr.data, ones, W, 256,
+                                                            mfp.engine.DEFAULT_LONG_ROW, mfp.engine.DEFAULT_CHUNK))
     assert m.get_effective_bandwidth() == 0.0  # csr_adaptive_spmv.cu:320 records 0 bytes
     m = mfp.gpu_ell_spmv(ell, A, col_ids, x_gpu, y, x, reference_answer)
     check(m, "GPU ELL", exact=reference_answer)

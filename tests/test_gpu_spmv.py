@@ -64,7 +64,9 @@ def test_csr_vector_bit_exact(gpu, name, dtype, lanes):
 @pytest.mark.parametrize("name", CASES)
 def test_csr_adaptive_bit_exact_default(gpu, name, dtype):
     y, (rp, ci, va, x), info = _run(gpu, name, dtype, "csr_adaptive")
-    ref = oracle.spmv_adaptive(rp, ci, va, x, info["nnz_per_wg"], 256, 4096, 2048)
+    E = gpu.engine
+    assert info["nnz_per_wg"] == E.DEFAULT_NNZ_PER_WG[np.dtype(dtype)]
+    ref = oracle.spmv_adaptive(rp, ci, va, x, info["nnz_per_wg"], 256, E.DEFAULT_LONG_ROW, E.DEFAULT_CHUNK)
     assert np.array_equal(y, ref)
     _tol_check(y, rp, ci, va, x)
 

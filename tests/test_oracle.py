@@ -246,6 +246,42 @@ def test_adaptive_long_row_chunks_add_left_to_right():
     assert y1[0] == np.float32(2.0 ** 24)
 
 
+@pytest.mark.parametrize("W", [64, 300, 2040, 4096])
+def test_adaptive_segmented_scan_covers_every_element(W):
+    # small integers: every summation order is exact, so the emulator must
+    # reproduce the exact row sums -- a dropped, repeated or misplaced element
+    # (lane ranges, row boundaries, scan carries, empty rows) changes them
+    rng = np.random.default_rng(W)
+    lens = rng.choice([0, 1, 2, 3, 9, 40, 300, 1500, 5000], 4000,
+                      p=[.15, .3, .2, .1, .1, .08, .04, .02, .01])
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    nnz = int(rp[-1])
+    ci = rng.integers(0, 4000, nnz).astype(np.uint32)
+    va = rng.integers(-3, 4, nnz).astype(np.float32)
+    x = rng.integers(-2, 3, 4000).astype(np.float32)
+    exact = np.zeros(4000)
+    np.add.at(exact, np.repeat(np.arange(4000), lens), va.astype(np.float64) * x[ci])
+    for long_row, chunk in ((1 << 30, 256), (4096, 2048), (512, 256)):
+        y = oracle.spmv_adaptive(rp, ci, va, x, W, 256, long_row, chunk)
+        assert np.array_equal(y.astype(np.float64), exact)
+
+
+def test_adaptive_scan_tree_order():
+    # block of two rows (4 + 1 elements, nnz_per_wg 5 -> one element per lane):
+    # row 0 spans lanes 0-3.  Hillis-Steele over the carried partials:
+    # step 1: s1 = p0 + p1, s2 = p1 + p2; step 2: s2 = s0 + s2 = p0 + (p1 + p2);
+    # lane 3 ends the row: y0 = s2 + p3.  With p = [2^24, 1, 1, 0] that is
+    # 2^24 + 2 (exact); left to right would round twice to 2^24.
+    v = [2.0 ** 24, 1, 1, 0, 0]
+    rp = np.array([0, 4, 5], np.uint32)
+    ci = np.arange(5, dtype=np.uint32)
+    va = np.array(v, np.float32)
+    x = np.ones(5, np.float32)
+    y = oracle.spmv_adaptive(rp, ci, va, x, nnz_per_wg=5)
+    assert y[0] == np.float32(2.0 ** 24 + 2)
+    assert oracle.spmv_seq(rp, ci, va, x)[0] == np.float32(2.0 ** 24)
+
+
 def test_stencil2d_closed_form():
     # 5-point Laplacian (4 / -1) with x = 1: y = 0 inside, 1 on edges, 2 at corners
     n = 16
