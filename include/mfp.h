@@ -122,6 +122,25 @@ int mfp_matrix_view_get(mfp_matrix m, mfp_matrix_view *out);
 int mfp_matrix_download(mfp_matrix m, uint32_t *row_ptr, uint32_t *columns, void *values);
 int mfp_matrix_destroy(mfp_matrix m);
 
+/* Copy of a matrix with its values converted to dtype (from f32/f64 values;
+ * 16-bit types are rounded from the FP32 value, nearest-even).  No reference
+ * counterpart: the value-precision study. */
+int mfp_matrix_convert(mfp_matrix src, int dtype, mfp_matrix *out);
+
+/* FP64 oracle SpMV on the GPU: y64 = A x with the stored values and x
+ * widened to double (one wave per row, double fma), and absdot = sum |a x|
+ * per row (may be NULL). */
+int mfp_spmv_f64_oracle(mfp_matrix m, int x_dtype, const void *x, double *y64, double *absdot,
+                        mfp_stream s);
+typedef struct {
+  double rel_l2;      /* ||y - y64|| / ||y64|| */
+  double max_row_err; /* max |y - y64| / absdot (or / |y64| without absdot) */
+  double max_abs_err;
+} mfp_error_report;
+/* Residual of a device result y (f32 / f64) against device y64 / absdot. */
+int mfp_residual(uint32_t n, int y_dtype, const void *y, const double *y64, const double *absdot,
+                 mfp_error_report *out);
+
 /* get_rows_statistics (common/matrix_converter.cpp:95-127), on the device. */
 typedef struct {
   uint32_t min_elements_in_rows, max_elements_in_rows, avg_elements_in_rows;
@@ -143,7 +162,9 @@ typedef struct {
   uint32_t scoo_lanes;    /* SCOO lane_size */
   uint32_t scoo_threads;  /* SCOO workgroup size */
   uint64_t budget_bytes;  /* refuse layouts above this (0 = 90% of free HBM) */
+  uint32_t flags;         /* MFP_TUNE_* */
 } mfp_tuning;
+enum { MFP_TUNE_NO_OFFSET16 = 1 /* 16-bit-value ELL: keep 32-bit columns */ };
 
 typedef struct mfp_plan_s *mfp_plan;
 int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *out);
@@ -151,6 +172,33 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
                     mfp_stream s);
 /* y = A x on stream s (no host sync; graph-capturable). */
 int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream s);
+/* Y = A X for k right-hand sides (k in {1,2,4,8,16}); X is cols x k and Y
+ * rows x k, row-major.  k > 1 runs on CSR-vector plans (f32 / f64); each
+ * column is summed in the CSR-vector order.  No reference counterpart (the
+ * reference multiplies one vector). */
+int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream s);
+
+/* ---- batches: many matrices in one persistent launch ----------------------
+ * No reference counterpart.  The matrices (same value type, f32 / f64) are
+ * assembled on the device into one block-diagonal CSR; X and Y are the
+ * matrices' vectors stacked in order (row i of matrix j at row_offsets[j] + i
+ * of Y, column offsets likewise for X), k columns each, row-major. */
+typedef struct mfp_batch_s *mfp_batch;
+typedef struct mfp_graph_s *mfp_graph;
+int mfp_batch_create(mfp_batch *out, const mfp_matrix *matrices, uint32_t count, uint32_t lanes,
+                     const mfp_tuning *adaptive_tuning /* NULL = defaults */, mfp_stream s);
+/* offsets arrays get count + 1 entries (may be NULL) */
+int mfp_batch_info(mfp_batch b, uint32_t *rows, uint32_t *cols, uint64_t *nnz, uint32_t *row_offsets,
+                   uint32_t *col_offsets);
+/* the assembled block-diagonal matrix (owned by the batch) */
+int mfp_batch_matrix(mfp_batch b, mfp_matrix *out);
+/* one persistent launch: CSR-Adaptive for k = 1, CSR-vector SpMM for k > 1 */
+int mfp_batch_spmm(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_stream s);
+/* capture that launch sequence once in a HIP graph; replay with mfp_graph_launch */
+int mfp_batch_graph_create(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_graph *out);
+int mfp_graph_launch(mfp_graph g, mfp_stream s);
+int mfp_graph_destroy(mfp_graph g);
+int mfp_batch_destroy(mfp_batch b);
 
 typedef struct {
   int format, dtype;
@@ -164,6 +212,7 @@ typedef struct {
   uint32_t slices_count, slice_size, lane_size;
   uint32_t lanes;
   uint32_t nnz_per_wg;
+  uint32_t column_bytes;     /* bytes per stored column index (2 with 16-bit offsets) */
 } mfp_plan_info;
 int mfp_plan_get_info(mfp_plan p, mfp_plan_info *out);
 
@@ -175,6 +224,24 @@ enum {
 };
 int mfp_plan_array(mfp_plan p, int which, const void **dev_ptr, uint64_t *count);
 int mfp_plan_destroy(mfp_plan p);
+
+/* ---- autotuner (no reference counterpart) -----------------------------------
+ * Candidates sweep lanes per row (CSR-vector), the CSR-Adaptive block size,
+ * the ELL/COO split of the hybrids (row-length quantiles) and the SCOO slice
+ * size.  mfp_autotune_run builds and times each (median of reps event-timed
+ * SpMVs, a cache flush before each when flush != 0); a candidate that cannot
+ * be built keeps ms = +inf and its status. */
+typedef struct {
+  int format;
+  mfp_tuning tuning;
+  double ms;
+  int status;
+} mfp_candidate;
+/* format_mask bit f enables format f; out may be NULL to count */
+int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *out, uint32_t cap,
+                            uint32_t *n);
+int mfp_autotune_run(mfp_matrix m, const void *x, void *y, mfp_candidate *cands, uint32_t n,
+                     uint32_t reps, int flush);
 
 /* ---- host converters run on the device (compat layer) --------------------- */
 /* fill_row_blocks (gpu/csr_adaptive_spmv.cu:180-240), nnz_per_wg generalising

@@ -8,9 +8,9 @@ ImportError if it has not been built
 from . import _lib
 from ._lib import MfpBudgetError, MfpError
 
-from .engine import (FORMATS, LABELS, DeviceArray, DeviceMatrix, GenSpec, Plan, Timer,  # noqa: E402
-                     device_info, flush_cache, generate, generate_vector, init, synchronize,
-                     workload)
+from .engine import (FORMATS, LABELS, Batch, autotune, DeviceArray, DeviceMatrix, GenSpec, Graph,  # noqa: E402
+                     Plan, Timer, device_info, flush_cache, generate, generate_vector, init,
+                     residual, synchronize, workload)
 from .reference_api import (compare_results, coo_matrix_class, csr_matrix_class,  # noqa: E402
                             ell_matrix_class, fill_row_blocks, get_rows_statistics,
                             gpu_coo_spmv, gpu_csr_adaptive_spmv, gpu_csr_spmv,
@@ -20,9 +20,9 @@ from .reference_api import (compare_results, coo_matrix_class, csr_matrix_class,
                             resizable_gpu_memory, scoo_geometry, scoo_matrix_class)
 
 __all__ = [
-    "MfpError", "MfpBudgetError", "FORMATS", "LABELS", "DeviceArray", "DeviceMatrix", "GenSpec",
-    "Plan", "Timer", "device_info", "flush_cache", "generate", "generate_vector", "init",
-    "synchronize", "workload", "compare_results", "coo_matrix_class", "csr_matrix_class",
+    "MfpError", "MfpBudgetError", "FORMATS", "LABELS", "Batch", "autotune", "DeviceArray", "DeviceMatrix",
+    "GenSpec", "Graph", "Plan", "Timer", "device_info", "flush_cache", "generate", "generate_vector",
+    "init", "residual", "synchronize", "workload", "compare_results", "coo_matrix_class", "csr_matrix_class",
     "ell_matrix_class", "fill_row_blocks", "get_rows_statistics", "gpu_coo_spmv",
     "gpu_csr_adaptive_spmv", "gpu_csr_spmv", "gpu_csr_vector_spmv", "gpu_ell_spmv",
     "gpu_hybrid_atomic_spmv", "gpu_hybrid_spmv", "gpu_scoo_spmv", "hybrid_matrix_class",

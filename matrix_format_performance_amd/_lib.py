@@ -54,14 +54,26 @@ class tuning(C.Structure):
     _fields_ = [("lanes", u32), ("nnz_per_wg", u32), ("long_row", u32), ("chunk", u32),
                 ("ell_width", u32), ("hyb_percent", dbl), ("hyb_width", u32), ("scoo_slices", u32),
                 ("scoo_slice_size", u32), ("scoo_lanes", u32), ("scoo_threads", u32),
-                ("budget_bytes", u64)]
+                ("budget_bytes", u64), ("flags", u32)]
+
+
+MFP_TUNE_NO_OFFSET16 = 1
+
+
+class candidate(C.Structure):
+    _fields_ = [("format", i32), ("tuning", tuning), ("ms", dbl), ("status", i32)]
+
+
+class error_report(C.Structure):
+    _fields_ = [("rel_l2", dbl), ("max_row_err", dbl), ("max_abs_err", dbl)]
 
 
 class plan_info(C.Structure):
     _fields_ = [("format", i32), ("dtype", i32), ("rows", u32), ("cols", u32), ("nnz", u64),
                 ("device_bytes", u64), ("stored_elements", u64), ("ell_width", u32),
                 ("coo_elements", u64), ("blocks", u32), ("chunks", u32), ("slices_count", u32),
-                ("slice_size", u32), ("lane_size", u32), ("lanes", u32), ("nnz_per_wg", u32)]
+                ("slice_size", u32), ("lane_size", u32), ("lanes", u32), ("nnz_per_wg", u32),
+                ("column_bytes", u32)]
 
 
 class measurement(C.Structure):
@@ -130,12 +142,26 @@ SIGNATURES = {
     "mfp_matrix_download": (i32, [vp, vp, vp, vp]),
     "mfp_matrix_destroy": (i32, [vp]),
     "mfp_matrix_row_stats": (i32, [vp, P(row_stats)]),
+    "mfp_matrix_convert": (i32, [vp, i32, P(vp)]),
+    "mfp_spmv_f64_oracle": (i32, [vp, i32, vp, vp, vp, vp]),
+    "mfp_residual": (i32, [u32, i32, vp, vp, vp, P(error_report)]),
     "mfp_tuning_default": (i32, [vp, i32, P(tuning)]),
     "mfp_plan_create": (i32, [P(vp), vp, i32, P(tuning), vp]),
     "mfp_plan_spmv": (i32, [vp, vp, vp, vp]),
+    "mfp_plan_spmm": (i32, [vp, u32, vp, vp, vp]),
+    "mfp_batch_create": (i32, [P(vp), P(vp), u32, u32, P(tuning), vp]),
+    "mfp_batch_info": (i32, [vp, P(u32), P(u32), P(u64), vp, vp]),
+    "mfp_batch_matrix": (i32, [vp, P(vp)]),
+    "mfp_batch_spmm": (i32, [vp, u32, vp, vp, vp]),
+    "mfp_batch_graph_create": (i32, [vp, u32, vp, vp, P(vp)]),
+    "mfp_graph_launch": (i32, [vp, vp]),
+    "mfp_graph_destroy": (i32, [vp]),
+    "mfp_batch_destroy": (i32, [vp]),
     "mfp_plan_get_info": (i32, [vp, P(plan_info)]),
     "mfp_plan_array": (i32, [vp, i32, P(vp), P(u64)]),
     "mfp_plan_destroy": (i32, [vp]),
+    "mfp_autotune_candidates": (i32, [vp, u32, vp, u32, P(u32)]),
+    "mfp_autotune_run": (i32, [vp, vp, vp, vp, u32, u32, i32]),
     "mfp_fill_row_blocks": (i32, [u32, vp, u32, vp, P(u32)]),
     "mfp_scoo_geometry": (i32, [i32, u32, u32, u64, P(u32), P(u32), P(u32)]),
     "mfp_compare_results": (i32, [i32, u32, vp, vp]),

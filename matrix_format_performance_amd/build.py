@@ -22,6 +22,9 @@ SOURCES = [
     "spmv.hip",
     "plan.hip",
     "reference_api.hip",
+    "precision.hip",
+    "batch.hip",
+    "autotune.cpp",
     "generate.cpp",
 ]
 

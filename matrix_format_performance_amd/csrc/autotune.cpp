@@ -1,0 +1,150 @@
+// Autotuner: candidate (format, tuning) sets for a matrix and their timing.
+//
+// Swept parameters: lanes per row (CSR-vector, 2..64), CSR-Adaptive block
+// size, ELL width (longest row), the ELL/COO split of the hybrids (row-length
+// quantiles), and the SCOO slice size (lane count filling a 32 KB LDS
+// accumulator).  Every candidate is built as a plan and timed with an event
+// pair (median of reps, each after an Infinity-Cache flush when asked).
+#include "../../include/mfp.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace {
+
+uint32_t prev_pow2(uint32_t v)
+{
+  uint32_t p = 1;
+  while ((p << 1) && (p << 1) <= v) p <<= 1;
+  return v ? p : 0;
+}
+
+void push(std::vector<mfp_candidate> &out, int fmt, const mfp_tuning &t)
+{
+  mfp_candidate c;
+  std::memset(&c, 0, sizeof(c));
+  c.format = fmt;
+  c.tuning = t;
+  c.ms = std::numeric_limits<double>::infinity();
+  c.status = -1;
+  out.push_back(c);
+}
+
+} // namespace
+
+extern "C" {
+
+int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *out, uint32_t cap,
+                            uint32_t *n)
+{
+  if (!m || !n) return MFP_ERR_ARG;
+  mfp_matrix_view v;
+  int st = mfp_matrix_view_get(m, &v);
+  if (st != MFP_OK) return st;
+  const bool lp = v.dtype == MFP_BF16 || v.dtype == MFP_F16;
+  // row-length quantiles from the row pointers
+  std::vector<uint32_t> rp((size_t)v.rows + 1), len(v.rows);
+  st = mfp_memcpy_d2h(rp.data(), v.row_ptr, rp.size() * 4);
+  if (st != MFP_OK) return st;
+  for (uint32_t r = 0; r < v.rows; r++) len[r] = rp[r + 1] - rp[r];
+  auto quant = [&](double q) -> uint32_t {
+    if (len.empty()) return 0;
+    std::vector<uint32_t> tmp(len);
+    const size_t k = std::min(tmp.size() - 1, (size_t)(q * (double)(tmp.size() - 1)));
+    std::nth_element(tmp.begin(), tmp.begin() + k, tmp.end());
+    return tmp[k];
+  };
+  const uint32_t avg = v.rows ? (uint32_t)((v.nnz + v.rows - 1) / v.rows) : 1;
+
+  std::vector<mfp_candidate> c;
+  auto want = [&](int f) { return (format_mask >> f) & 1u; };
+  for (int f = 0; f < MFP_FORMAT_COUNT; f++) {
+    if (!want(f)) continue;
+    if (lp && f != MFP_CSR_VECTOR && f != MFP_ELL) continue;
+    mfp_tuning t;
+    st = mfp_tuning_default(m, f, &t);
+    if (st != MFP_OK) return st;
+    switch (f) {
+    case MFP_CSR_VECTOR:
+      for (uint32_t l = 2; l <= 64; l <<= 1) {
+        t.lanes = l;
+        push(c, f, t);
+      }
+      break;
+    case MFP_CSR_ADAPTIVE: {
+      const uint32_t base = t.nnz_per_wg;
+      for (uint32_t w : {base / 2, base, base * 2}) {
+        t.nnz_per_wg = w;
+        push(c, f, t);
+      }
+      break;
+    }
+    case MFP_SCOO: {
+      const uint32_t acc = (v.dtype == MFP_F64 ? 4096u : 8192u);
+      for (uint32_t ss : {256u, 512u, 1024u, 2048u, 4096u}) {
+        if (v.rows && ss > std::max<uint32_t>(256, v.rows)) break;
+        t.scoo_slice_size = ss;
+        t.scoo_lanes = std::max<uint32_t>(1, std::min<uint32_t>(32, prev_pow2(acc / ss)));
+        t.scoo_slices = v.rows ? (v.rows + ss - 1) / ss : 1;
+        push(c, f, t);
+      }
+      break;
+    }
+    case MFP_HYB:
+    case MFP_HYB_ATOMIC: {
+      std::vector<uint32_t> ws = {avg, quant(0.75), quant(0.9), quant(0.99)};
+      std::sort(ws.begin(), ws.end());
+      ws.erase(std::unique(ws.begin(), ws.end()), ws.end());
+      for (uint32_t w : ws) {
+        if (w == 0) continue;
+        t.hyb_width = w;
+        push(c, f, t);
+      }
+      break;
+    }
+    default: push(c, f, t); break;
+    }
+  }
+  *n = (uint32_t)c.size();
+  if (out) std::memcpy(out, c.data(), std::min<size_t>(cap, c.size()) * sizeof(mfp_candidate));
+  return MFP_OK;
+}
+
+int mfp_autotune_run(mfp_matrix m, const void *x, void *y, mfp_candidate *cands, uint32_t n,
+                     uint32_t reps, int flush)
+{
+  if (!m || !x || !y || (!cands && n)) return MFP_ERR_ARG;
+  if (reps == 0) reps = 1;
+  mfp_timer tm = nullptr;
+  int st = mfp_timer_create(&tm);
+  if (st != MFP_OK) return st;
+  for (uint32_t i = 0; i < n; i++) {
+    mfp_candidate &c = cands[i];
+    mfp_plan p = nullptr;
+    c.status = mfp_plan_create(&p, m, c.format, &c.tuning, nullptr);
+    c.ms = std::numeric_limits<double>::infinity();
+    if (c.status != MFP_OK) continue;
+    std::vector<double> t;
+    for (uint32_t r = 0; r <= reps && c.status == MFP_OK; r++) { // first run warms the code path
+      if (flush) c.status = mfp_flush_cache(1ull << 30, nullptr);
+      if (c.status == MFP_OK) c.status = mfp_timer_start(tm, nullptr);
+      if (c.status == MFP_OK) c.status = mfp_plan_spmv(p, x, y, nullptr);
+      if (c.status == MFP_OK) c.status = mfp_timer_stop(tm, nullptr);
+      float ms = 0;
+      if (c.status == MFP_OK) c.status = mfp_timer_elapsed_ms(tm, &ms);
+      if (r > 0) t.push_back(ms);
+    }
+    mfp_plan_destroy(p);
+    if (c.status == MFP_OK && !t.empty()) {
+      std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+      c.ms = t[t.size() / 2];
+    }
+  }
+  mfp_timer_destroy(tm);
+  return MFP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,316 @@
+// Multiple right-hand sides (SpMM) and batched matrices.
+//
+// SpMM: Y = A X with X (cols x k) and Y (rows x k) row-major.  G lanes per
+// row (CSR-vector order, per column of X: lane-strided fma, halving tree),
+// each lane reading the k entries of an X row with 16-byte loads.
+//
+// Batch: many matrices of uneven sizes become one block-diagonal CSR in HBM
+// (row pointers offset by the preceding nnz, columns by the preceding
+// columns); X / Y are the matrices' vectors stacked in the same order.  One
+// persistent launch (CSR-Adaptive for k = 1, SpMM otherwise) covers the whole
+// batch, and the launch sequence can be captured once in a HIP graph and
+// replayed.
+#include "kernels.h"
+#include "mfp_internal.h"
+#include "wave64.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace mfp {
+
+template <typename T, int G, int KV, bool PERSISTENT>
+__global__ __launch_bounds__(256) void k_spmm_vector(uint32_t rows, const uint32_t *__restrict__ rp,
+                                                     const uint32_t *__restrict__ ci,
+                                                     const T *__restrict__ a, const T *__restrict__ X,
+                                                     T *__restrict__ Y)
+{
+  const uint32_t lane = threadIdx.x % G;
+  const uint32_t rows_per_block = 256u / G;
+  const uint32_t nblk = (rows + rows_per_block - 1) / rows_per_block;
+  for (uint32_t blk = blockIdx.x; blk < nblk; blk += (PERSISTENT ? gridDim.x : nblk)) {
+    const uint32_t row = blk * rows_per_block + threadIdx.x / G;
+    T acc[KV];
+#pragma unroll
+    for (int j = 0; j < KV; j++) acc[j] = 0;
+    if (row < rows) {
+      const uint32_t b = rp[row], e = rp[row + 1];
+      for (uint32_t i = b + lane; i < e; i += G) {
+        const T av = ldg_stream(a + i);
+        const T *xr = X + (uint64_t)ldg_stream(ci + i) * KV;
+        T xv[KV];
+#pragma unroll
+        for (int j = 0; j < KV; j++) xv[j] = xr[j];
+#pragma unroll
+        for (int j = 0; j < KV; j++) acc[j] = fma_t(av, xv[j], acc[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < KV; j++) acc[j] = group_reduce<G>(acc[j]);
+    if (lane == 0 && row < rows) {
+#pragma unroll
+      for (int j = 0; j < KV; j++) Y[(uint64_t)row * KV + j] = acc[j];
+    }
+    if (!PERSISTENT) break;
+  }
+}
+
+template <typename T, int KV, bool P>
+static int spmm_g(uint32_t lanes, uint32_t rows, const uint32_t *rp, const uint32_t *ci, const T *a,
+                  const T *X, T *Y, uint32_t grid, hipStream_t s)
+{
+  switch (lanes) {
+  case 2: k_spmm_vector<T, 2, KV, P><<<grid, 256, 0, s>>>(rows, rp, ci, a, X, Y); break;
+  case 4: k_spmm_vector<T, 4, KV, P><<<grid, 256, 0, s>>>(rows, rp, ci, a, X, Y); break;
+  case 8: k_spmm_vector<T, 8, KV, P><<<grid, 256, 0, s>>>(rows, rp, ci, a, X, Y); break;
+  case 16: k_spmm_vector<T, 16, KV, P><<<grid, 256, 0, s>>>(rows, rp, ci, a, X, Y); break;
+  case 32: k_spmm_vector<T, 32, KV, P><<<grid, 256, 0, s>>>(rows, rp, ci, a, X, Y); break;
+  case 64: k_spmm_vector<T, 64, KV, P><<<grid, 256, 0, s>>>(rows, rp, ci, a, X, Y); break;
+  default: set_error("lanes must be a power of two in [2, 64]"); return MFP_ERR_ARG;
+  }
+  return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
+}
+
+template <typename T, bool P>
+static int spmm_k(uint32_t k, uint32_t lanes, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
+                  const T *a, const T *X, T *Y, uint32_t grid, hipStream_t s)
+{
+  switch (k) {
+  case 1: return spmm_g<T, 1, P>(lanes, rows, rp, ci, a, X, Y, grid, s);
+  case 2: return spmm_g<T, 2, P>(lanes, rows, rp, ci, a, X, Y, grid, s);
+  case 4: return spmm_g<T, 4, P>(lanes, rows, rp, ci, a, X, Y, grid, s);
+  case 8: return spmm_g<T, 8, P>(lanes, rows, rp, ci, a, X, Y, grid, s);
+  case 16: return spmm_g<T, 16, P>(lanes, rows, rp, ci, a, X, Y, grid, s);
+  default: set_error("k must be 1, 2, 4, 8 or 16"); return MFP_ERR_ARG;
+  }
+}
+
+int launch_spmm_vector(int dtype, uint32_t k, uint32_t lanes, uint32_t rows, const uint32_t *rp,
+                       const uint32_t *ci, const void *a, const void *X, void *Y, bool persistent,
+                       hipStream_t s)
+{
+  if (rows == 0) return MFP_OK;
+  const uint32_t nblk = ceil_div_u32((uint64_t)rows * lanes, 256);
+  uint32_t grid = nblk;
+  if (persistent) grid = std::min<uint32_t>(nblk, (uint32_t)std::max(1, props().cu_count) * 8u);
+  if (dtype == MFP_F32)
+    return persistent ? spmm_k<float, true>(k, lanes, rows, rp, ci, (const float *)a, (const float *)X, (float *)Y, grid, s)
+                      : spmm_k<float, false>(k, lanes, rows, rp, ci, (const float *)a, (const float *)X, (float *)Y, grid, s);
+  return persistent ? spmm_k<double, true>(k, lanes, rows, rp, ci, (const double *)a, (const double *)X, (double *)Y, grid, s)
+                    : spmm_k<double, false>(k, lanes, rows, rp, ci, (const double *)a, (const double *)X, (double *)Y, grid, s);
+}
+
+// ---- batch assembly ---------------------------------------------------------------
+__global__ void k_offset_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t n,
+                             uint32_t add)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = in[i] + add;
+}
+
+int offset_copy_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t add, hipStream_t s)
+{
+  if (n == 0) return MFP_OK;
+  k_offset_u32<<<std::min<uint32_t>(ceil_div_u32(n, 256), 65536), 256, 0, s>>>(in, out, n, add);
+  return hipGetLastError() == hipSuccess ? MFP_OK : MFP_ERR_HIP;
+}
+
+} // namespace mfp
+
+using namespace mfp;
+
+#include "objects.h"
+
+struct mfp_batch_s {
+  uint32_t count = 0;
+  int dtype = MFP_F32;
+  mfp_matrix big = nullptr; // block-diagonal CSR
+  std::vector<uint32_t> row_off, col_off;
+  mfp_plan adaptive = nullptr; // k = 1
+  uint32_t lanes = 8;          // k > 1
+};
+
+struct mfp_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int batch_launch(mfp_batch b, uint32_t k, const void *X, void *Y, hipStream_t s)
+{
+  const mfp_matrix m = b->big;
+  if (k == 1) {
+    const mfp_plan p = b->adaptive;
+    adaptive_args g;
+    g.rows = m->rows;
+    g.rp = m->row_ptr;
+    g.ci = m->columns;
+    g.blocks = p->ad.blocks;
+    g.nblocks = p->ad.nblocks;
+    g.nnz_per_wg = p->t.nnz_per_wg;
+    g.long_row = p->t.long_row;
+    g.chunks = p->ad.chunks;
+    g.nchunks = p->ad.nchunks;
+    g.chunk = p->t.chunk;
+    g.partials = p->ad.partials;
+    g.persistent = true;
+    return launch_csr_adaptive(m->dtype, g, m->values, X, Y, s);
+  }
+  return launch_spmm_vector(m->dtype, k, b->lanes, m->rows, m->row_ptr, m->columns, m->values, X, Y,
+                            true, s);
+}
+
+extern "C" {
+
+int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream sv)
+{
+  MFP_ARG(p && X && Y, "null pointer");
+  if (k == 1) return mfp_plan_spmv(p, X, Y, sv);
+  MFP_ARG(p->format == MFP_CSR_VECTOR, "SpMM with k > 1 runs on CSR-vector plans");
+  MFP_ARG(p->dtype == MFP_F32 || p->dtype == MFP_F64, "SpMM values must be f32 or f64");
+  const mfp_matrix m = p->m;
+  return launch_spmm_vector(p->dtype, k, p->t.lanes, m->rows, m->row_ptr, m->columns, m->values, X, Y,
+                            false, as_stream(sv));
+}
+
+int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint32_t lanes,
+                     const mfp_tuning *adaptive_tuning, mfp_stream sv)
+{
+  MFP_ARG(out && ms && count > 0, "null pointer or empty batch");
+  MFP_ARG(lanes >= 2 && lanes <= 64 && (lanes & (lanes - 1)) == 0, "lanes");
+  MFP_CHECK(ensure_init());
+  hipStream_t s = as_stream(sv);
+  const int dt = ms[0]->dtype;
+  MFP_ARG(dt == MFP_F32 || dt == MFP_F64, "batched values must be f32 or f64");
+  uint64_t rows = 0, cols = 0, nnz = 0;
+  auto *b = new mfp_batch_s;
+  b->count = count;
+  b->dtype = dt;
+  b->lanes = lanes;
+  for (uint32_t i = 0; i < count; i++) {
+    if (!ms[i] || ms[i]->dtype != dt) {
+      delete b;
+      set_error("every batched matrix needs the same value type");
+      return MFP_ERR_ARG;
+    }
+    b->row_off.push_back((uint32_t)rows);
+    b->col_off.push_back((uint32_t)cols);
+    rows += ms[i]->rows;
+    cols += ms[i]->cols;
+    nnz += ms[i]->nnz;
+  }
+  if (rows >= (1ull << 32) || cols >= (1ull << 32) || nnz >= (1ull << 32)) {
+    delete b;
+    set_error("batch exceeds 32-bit rows / cols / nnz");
+    return MFP_ERR_ARG;
+  }
+  b->row_off.push_back((uint32_t)rows);
+  b->col_off.push_back((uint32_t)cols);
+  auto *m = new mfp_matrix_s;
+  m->dtype = dt;
+  m->rows = (uint32_t)rows;
+  m->cols = (uint32_t)cols;
+  m->nnz = nnz;
+  b->big = m;
+  const size_t vs = dtype_size(dt);
+  int st = dmalloc((void **)&m->row_ptr, (rows + 1) * 4);
+  if (st == MFP_OK) st = dmalloc((void **)&m->columns, nnz * 4);
+  if (st == MFP_OK) st = dmalloc(&m->values, nnz * vs);
+  uint64_t e0 = 0;
+  for (uint32_t i = 0; i < count && st == MFP_OK; i++) {
+    const mfp_matrix a = ms[i];
+    // rows of matrix i: row_ptr + nnz offset (its last entry is the next one's first)
+    st = offset_copy_u32(a->row_ptr, m->row_ptr + b->row_off[i], (uint64_t)a->rows + 1, (uint32_t)e0, s);
+    if (st == MFP_OK) st = offset_copy_u32(a->columns, m->columns + e0, a->nnz, b->col_off[i], s);
+    if (st == MFP_OK && a->nnz)
+      st = hipMemcpyAsync((char *)m->values + e0 * vs, a->values, a->nnz * vs, hipMemcpyDeviceToDevice, s) ==
+                   hipSuccess ? MFP_OK : MFP_ERR_HIP;
+    e0 += a->nnz;
+  }
+  if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
+  if (st == MFP_OK) st = mfp_plan_create(&b->adaptive, m, MFP_CSR_ADAPTIVE, adaptive_tuning, sv);
+  if (st != MFP_OK) {
+    mfp_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return MFP_OK;
+}
+
+int mfp_batch_info(mfp_batch b, uint32_t *rows, uint32_t *cols, uint64_t *nnz, uint32_t *row_offsets,
+                   uint32_t *col_offsets)
+{
+  MFP_ARG(b, "null batch");
+  if (rows) *rows = b->big->rows;
+  if (cols) *cols = b->big->cols;
+  if (nnz) *nnz = b->big->nnz;
+  if (row_offsets) std::memcpy(row_offsets, b->row_off.data(), b->row_off.size() * 4);
+  if (col_offsets) std::memcpy(col_offsets, b->col_off.data(), b->col_off.size() * 4);
+  return MFP_OK;
+}
+
+int mfp_batch_matrix(mfp_batch b, mfp_matrix *out)
+{
+  MFP_ARG(b && out, "null pointer");
+  *out = b->big;
+  return MFP_OK;
+}
+
+int mfp_batch_spmm(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_stream sv)
+{
+  MFP_ARG(b && X && Y, "null pointer");
+  return batch_launch(b, k, X, Y, as_stream(sv));
+}
+
+int mfp_batch_graph_create(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_graph *out)
+{
+  MFP_ARG(b && X && Y && out, "null pointer");
+  MFP_CHECK(ensure_init());
+  hipStream_t cs = nullptr;
+  MFP_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  auto *g = new mfp_graph_s;
+  int st = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess ? MFP_OK : MFP_ERR_HIP;
+  if (st == MFP_OK) {
+    st = batch_launch(b, k, X, Y, cs);
+    const hipError_t e = hipStreamEndCapture(cs, &g->graph);
+    if (st == MFP_OK && e != hipSuccess) st = MFP_ERR_HIP;
+  }
+  if (st == MFP_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    st = MFP_ERR_HIP;
+  (void)hipStreamDestroy(cs);
+  if (st != MFP_OK) {
+    set_error("graph capture of the batched launch failed");
+    mfp_graph_destroy(g);
+    return st;
+  }
+  *out = g;
+  return MFP_OK;
+}
+
+int mfp_graph_launch(mfp_graph g, mfp_stream sv)
+{
+  MFP_ARG(g && g->exec, "null graph");
+  MFP_HIP(hipGraphLaunch(g->exec, as_stream(sv)));
+  return MFP_OK;
+}
+
+int mfp_graph_destroy(mfp_graph g)
+{
+  if (!g) return MFP_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return MFP_OK;
+}
+
+int mfp_batch_destroy(mfp_batch b)
+{
+  if (!b) return MFP_OK;
+  mfp_plan_destroy(b->adaptive);
+  mfp_matrix_destroy(b->big);
+  delete b;
+  return MFP_OK;
+}
+
+} // extern "C"

@@ -18,6 +18,7 @@ struct adaptive_args {
   uint32_t nchunks = 0;
   uint32_t chunk = 0;
   void *partials = nullptr; // nchunks values
+  bool persistent = false;  // fixed grid walking the items (batched launches)
 };
 
 // CSR-Adaptive defaults: a row block holds up to this many nonzeros, which a
@@ -55,6 +56,11 @@ struct adaptive_layout {
 int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t nnz_per_wg,
                           uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s);
 void free_adaptive_layout(adaptive_layout *l);
+
+int launch_spmm_vector(int dtype, uint32_t k, uint32_t lanes, uint32_t rows, const uint32_t *rp,
+                       const uint32_t *ci, const void *a, const void *X, void *Y, bool persistent,
+                       hipStream_t s);
+int offset_copy_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t add, hipStream_t s);
 
 int csr_from_triplets_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz,
                              const uint32_t *tr_rows, const uint32_t *tr_cols,

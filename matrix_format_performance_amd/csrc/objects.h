@@ -1,0 +1,36 @@
+// Opaque C-ABI objects (matrix, plan), shared by plan.hip and batch.hip.
+#pragma once
+
+#include "kernels.h"
+#include "mfp_internal.h"
+
+struct mfp_matrix_s {
+  int dtype = MFP_F32;
+  uint32_t rows = 0, cols = 0;
+  uint64_t nnz = 0;
+  uint32_t *row_ptr = nullptr, *columns = nullptr;
+  void *values = nullptr;
+  bool stats_valid = false;
+  mfp_row_stats stats{};
+};
+
+struct mfp_plan_s {
+  int format = 0, dtype = 0;
+  mfp_matrix m = nullptr;
+  mfp_tuning t{};
+  uint32_t ell_w = 0;
+  void *ell_val = nullptr;
+  uint32_t *ell_col = nullptr;
+  uint64_t ncoo = 0;
+  void *coo_val = nullptr;
+  uint32_t *coo_col = nullptr, *coo_row = nullptr;
+  uint32_t slices = 0, ss = 0, L = 0, scoo_threads = 512;
+  uint32_t *sp = nullptr;
+  adaptive_layout ad;
+  uint64_t device_bytes = 0;
+  // 16-bit-value ELL: columns as u16 offsets from base[row] when off16
+  bool off16 = false;
+  void *lp_cols = nullptr;
+  uint32_t *lp_base = nullptr;
+};
+

@@ -77,31 +77,20 @@ int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t
 
 using namespace mfp;
 
-struct mfp_matrix_s {
-  int dtype = MFP_F32;
-  uint32_t rows = 0, cols = 0;
-  uint64_t nnz = 0;
-  uint32_t *row_ptr = nullptr, *columns = nullptr;
-  void *values = nullptr;
-  bool stats_valid = false;
-  mfp_row_stats stats{};
-};
+#include "objects.h"
 
-struct mfp_plan_s {
-  int format = 0, dtype = 0;
-  mfp_matrix m = nullptr;
-  mfp_tuning t{};
-  uint32_t ell_w = 0;
-  void *ell_val = nullptr;
-  uint32_t *ell_col = nullptr;
-  uint64_t ncoo = 0;
-  void *coo_val = nullptr;
-  uint32_t *coo_col = nullptr, *coo_row = nullptr;
-  uint32_t slices = 0, ss = 0, L = 0, scoo_threads = 512;
-  uint32_t *sp = nullptr;
-  adaptive_layout ad;
-  uint64_t device_bytes = 0;
-};
+namespace mfp {
+int launch_csr_vector_lp(int dt, uint32_t lanes, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
+                         const void *a, const void *x, void *y, hipStream_t s);
+int launch_ell_lp(int dt, bool off16, uint32_t rows, uint32_t w, const void *cols, const uint32_t *base,
+                  const void *ev, const void *x, void *y, hipStream_t s);
+int row_span_max(uint32_t rows, const uint32_t *rp, const uint32_t *ci, uint32_t *out, hipStream_t s);
+int ell_lp_build(uint32_t rows, const uint32_t *rp, const uint32_t *ci, const void *a, uint32_t w,
+                 bool off16, void *ev, void *cols, uint32_t *base, hipStream_t s);
+int to_lp_values(const void *in, int din, void *out, int dout, uint64_t n, hipStream_t s);
+} // namespace mfp
+
+static bool is_lp(int dtype) { return dtype == MFP_BF16 || dtype == MFP_F16; }
 
 static int matrix_stats(mfp_matrix m, mfp_row_stats *out, hipStream_t s)
 {
@@ -152,9 +141,15 @@ static int plan_free_arrays(mfp_plan p)
   dfree(p->coo_col);
   dfree(p->coo_row);
   dfree(p->sp);
+  dfree(p->lp_cols);
+  dfree(p->lp_base);
   free_adaptive_layout(&p->ad);
   return MFP_OK;
 }
+
+// Plans over 16-bit values: CSR-vector and ELL with Dot2-compensated FP32
+// accumulation.
+static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipStream_t s);
 
 static uint64_t budget_of(const mfp_tuning &t)
 {
@@ -164,6 +159,44 @@ static uint64_t budget_of(const mfp_tuning &t)
   return (uint64_t)(0.9 * (double)fr);
 }
 
+static int matrix_stats(mfp_matrix m, mfp_row_stats *out, hipStream_t s);
+
+static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipStream_t s)
+{
+  if (p->format == MFP_CSR_VECTOR) {
+    if (t.lanes < 2 || t.lanes > 64 || (t.lanes & (t.lanes - 1))) {
+      set_error("lanes must be a power of two in [2, 64]");
+      return MFP_ERR_ARG;
+    }
+    return MFP_OK;
+  }
+  if (p->format != MFP_ELL) {
+    set_error("16-bit values are available in the CSR-vector and ELL formats");
+    return MFP_ERR_ARG;
+  }
+  mfp_row_stats rs;
+  MFP_CHECK(matrix_stats(m, &rs, s));
+  const uint32_t w = rs.max_elements_in_rows;
+  uint32_t span = 0;
+  MFP_CHECK(row_span_max(m->rows, m->row_ptr, m->columns, &span, s));
+  p->off16 = span < 65536u && !(t.flags & MFP_TUNE_NO_OFFSET16);
+  const uint64_t slots = (uint64_t)m->rows * w;
+  const uint64_t need = slots * (2 + (p->off16 ? 2 : 4)) + (p->off16 ? (uint64_t)m->rows * 4 : 0);
+  if (need > budget_of(t)) {
+    set_error("ELL part needs " + std::to_string(need) + " bytes, above the budget");
+    return MFP_ERR_BUDGET;
+  }
+  p->ell_w = w;
+  MFP_CHECK(dmalloc(&p->ell_val, slots * 2));
+  MFP_CHECK(dmalloc(&p->lp_cols, slots * (p->off16 ? 2 : 4)));
+  if (p->off16) MFP_CHECK(dmalloc((void **)&p->lp_base, (uint64_t)m->rows * 4));
+  MFP_CHECK(ell_lp_build(m->rows, m->row_ptr, m->columns, m->values, w, p->off16, p->ell_val,
+                         p->lp_cols, p->lp_base, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  p->device_bytes = need;
+  return MFP_OK;
+}
+
 extern "C" {
 
 int mfp_matrix_from_host_csr(mfp_matrix *out, int dtype, uint32_t rows, uint32_t cols,
@@ -171,7 +204,7 @@ int mfp_matrix_from_host_csr(mfp_matrix *out, int dtype, uint32_t rows, uint32_t
                              const void *values)
 {
   MFP_ARG(out && row_ptr, "null pointer");
-  MFP_ARG(dtype == MFP_F32 || dtype == MFP_F64, "dtype must be f32 or f64");
+  MFP_ARG(dtype_size(dtype) != 0, "dtype");
   MFP_ARG(nnz < (1ull << 32), "nnz must fit in 32 bits");
   MFP_ARG(row_ptr[0] == 0 && row_ptr[rows] == nnz, "row_ptr must start at 0 and end at nnz");
   MFP_CHECK(ensure_init());
@@ -268,6 +301,34 @@ int mfp_matrix_download(mfp_matrix m, uint32_t *row_ptr, uint32_t *columns, void
   return MFP_OK;
 }
 
+int mfp_matrix_convert(mfp_matrix src, int dtype, mfp_matrix *out)
+{
+  MFP_ARG(src && out, "null pointer");
+  MFP_ARG(dtype_size(dtype) != 0, "dtype");
+  MFP_ARG(src->dtype == MFP_F32 || src->dtype == MFP_F64, "source values must be f32 or f64");
+  MFP_CHECK(ensure_init());
+  auto *m = new mfp_matrix_s;
+  m->dtype = dtype;
+  m->rows = src->rows;
+  m->cols = src->cols;
+  m->nnz = src->nnz;
+  int st = dmalloc((void **)&m->row_ptr, ((uint64_t)m->rows + 1) * 4);
+  if (st == MFP_OK) st = dmalloc((void **)&m->columns, m->nnz * 4);
+  if (st == MFP_OK) st = dmalloc(&m->values, m->nnz * dtype_size(dtype));
+  if (st == MFP_OK) st = mfp_memcpy_d2d(m->row_ptr, src->row_ptr, ((uint64_t)m->rows + 1) * 4, nullptr);
+  if (st == MFP_OK) st = mfp_memcpy_d2d(m->columns, src->columns, m->nnz * 4, nullptr);
+  if (st == MFP_OK)
+    st = is_lp(dtype) ? to_lp_values(src->values, src->dtype, m->values, dtype, m->nnz, nullptr)
+                      : convert_values(src->dtype, src->values, dtype, m->values, m->nnz, nullptr);
+  if (st == MFP_OK && hipDeviceSynchronize() != hipSuccess) st = MFP_ERR_HIP;
+  if (st != MFP_OK) {
+    mfp_matrix_destroy(m);
+    return st;
+  }
+  *out = m;
+  return MFP_OK;
+}
+
 int mfp_matrix_destroy(mfp_matrix m)
 {
   if (!m) return MFP_OK;
@@ -329,6 +390,12 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
     delete p;
     return code;
   };
+  if (is_lp(m->dtype)) {
+    st = plan_create_lp(p, m, t, s);
+    if (st != MFP_OK) return fail(st);
+    *out = p;
+    return MFP_OK;
+  }
   switch (format) {
   case MFP_CSR_SCALAR: break;
   case MFP_CSR_VECTOR:
@@ -424,6 +491,11 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
   hipStream_t s = as_stream(sv);
   const mfp_matrix m = p->m;
   const int dt = p->dtype;
+  if (is_lp(dt)) {
+    if (p->format == MFP_CSR_VECTOR)
+      return launch_csr_vector_lp(dt, p->t.lanes, m->rows, m->row_ptr, m->columns, m->values, x, y, s);
+    return launch_ell_lp(dt, p->off16, m->rows, p->ell_w, p->lp_cols, p->lp_base, p->ell_val, x, y, s);
+  }
   switch (p->format) {
   case MFP_CSR_SCALAR:
     return launch_csr_scalar(dt, m->rows, m->row_ptr, m->columns, m->values, x, y, s);
@@ -484,6 +556,7 @@ int mfp_plan_get_info(mfp_plan p, mfp_plan_info *o)
   o->lane_size = p->L;
   o->lanes = p->t.lanes;
   o->nnz_per_wg = p->t.nnz_per_wg;
+  o->column_bytes = p->off16 ? 2u : 4u;
   return MFP_OK;
 }
 

@@ -1,0 +1,411 @@
+// Precision paths.
+//
+// * FP64 oracle SpMV on the GPU (y64 = A x with the stored values widened to
+//   double) and the residual report of an SpMV result against it.
+// * Low-precision value storage: BF16 / FP16 values, FP32 x and y, FP32
+//   accumulation compensated with the Dot2 scheme of Ogita, Rump and Oishi,
+//   "Accurate Sum and Dot Product" (SIAM J. Sci. Comput. 26(6), 2005):
+//   TwoProduct by fma (p = a*x, e = fma(a, x, -p)) and Knuth's TwoSum
+//   (s' = s + p, z = s' - s, t = (s - (s' - z)) + (p - z)); the error terms
+//   are summed in a second accumulator and added at the end.  Lane partials
+//   (s, c) are combined with TwoSum as well.
+// * ELL with 16-bit column offsets: when every row's column span is below
+//   65536, columns are stored as (row base: u32, offset: u16), which with
+//   16-bit values makes 4 bytes per stored entry instead of 8.
+#include "kernels.h"
+#include "mfp_internal.h"
+#include "wave64.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace mfp {
+
+// ---- value loads -------------------------------------------------------------
+__device__ __forceinline__ float lp_to_f32_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ float lp_to_f32_f16(uint16_t h)
+{
+  return __half2float(__ushort_as_half(h));
+}
+
+template <int DT>
+__device__ __forceinline__ float lp_val(uint16_t v)
+{
+  if constexpr (DT == MFP_BF16)
+    return lp_to_f32_bf16(v);
+  else
+    return lp_to_f32_f16(v);
+}
+
+template <typename T>
+__device__ __forceinline__ double widen(const T *p, uint64_t i, int dt)
+{
+  return (double)p[i];
+}
+
+__device__ __forceinline__ double value_f64(const void *vals, uint64_t i, int dt)
+{
+  switch (dt) {
+  case MFP_F32: return (double)((const float *)vals)[i];
+  case MFP_F64: return ((const double *)vals)[i];
+  case MFP_BF16: return (double)lp_to_f32_bf16(((const uint16_t *)vals)[i]);
+  default: return (double)lp_to_f32_f16(((const uint16_t *)vals)[i]);
+  }
+}
+
+// ---- compensated accumulation (Dot2) ---------------------------------------------
+struct comp {
+  float s, c;
+};
+
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &t)
+{
+  s = a + b;
+  const float z = s - a;
+  t = (a - (s - z)) + (b - z);
+}
+
+__device__ __forceinline__ void dot2_add(comp &acc, float a, float x)
+{
+  const float p = a * x;
+  const float e = __builtin_fmaf(a, x, -p);
+  float s, t;
+  two_sum(acc.s, p, s, t);
+  acc.s = s;
+  acc.c = acc.c + (t + e);
+}
+
+__device__ __forceinline__ comp comp_merge(comp u, comp v)
+{
+  float s, t;
+  two_sum(u.s, v.s, s, t);
+  return comp{s, (u.c + v.c) + t};
+}
+
+template <int G>
+__device__ __forceinline__ comp comp_group_reduce(comp v)
+{
+  if constexpr (G >= 64) v = comp_merge(v, comp{lane_down<32>(v.s), lane_down<32>(v.c)});
+  if constexpr (G >= 32) v = comp_merge(v, comp{lane_down<16>(v.s), lane_down<16>(v.c)});
+  if constexpr (G >= 16) v = comp_merge(v, comp{lane_down<8>(v.s), lane_down<8>(v.c)});
+  if constexpr (G >= 8) v = comp_merge(v, comp{lane_down<4>(v.s), lane_down<4>(v.c)});
+  if constexpr (G >= 4) v = comp_merge(v, comp{lane_down<2>(v.s), lane_down<2>(v.c)});
+  if constexpr (G >= 2) v = comp_merge(v, comp{lane_down<1>(v.s), lane_down<1>(v.c)});
+  return v;
+}
+
+// ---- FP64 oracle SpMV and residual ------------------------------------------------
+// One wave per row: 64 lanes strided, double fma, halving tree.
+__global__ __launch_bounds__(256) void k_spmv_f64(uint32_t rows, const uint32_t *__restrict__ rp,
+                                                  const uint32_t *__restrict__ ci, const void *vals,
+                                                  int vdt, const void *x, int xdt,
+                                                  double *__restrict__ y64, double *__restrict__ absdot)
+{
+  const uint32_t row = blockIdx.x * 4u + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63u;
+  double d = 0.0, ad = 0.0;
+  if (row < rows) {
+    for (uint32_t i = rp[row] + lane; i < rp[row + 1]; i += 64) {
+      const double p = value_f64(vals, i, vdt) * value_f64(x, ci[i], xdt);
+      d += p;
+      ad += fabs(p);
+    }
+  }
+  d = group_reduce<64>(d);
+  ad = group_reduce<64>(ad);
+  if (lane == 0 && row < rows) {
+    y64[row] = d;
+    if (absdot) absdot[row] = ad;
+  }
+}
+
+struct resid_acc {
+  double num, den;
+  unsigned long long max_norm_bits, max_abs_bits;
+  unsigned long long worst; // (bits of max_norm << 32) is not representable; row kept separately
+};
+
+__global__ void k_residual(uint32_t n, const void *y, int ydt, const double *__restrict__ y64,
+                           const double *__restrict__ absdot, resid_acc *acc)
+{
+  double num = 0, den = 0, mx = 0, mabs = 0;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const double v = value_f64(y, i, ydt);
+    const double d = v - y64[i];
+    num += d * d;
+    den += y64[i] * y64[i];
+    const double ad = fabs(d);
+    mabs = fmax(mabs, ad);
+    const double sc = absdot ? absdot[i] : fabs(y64[i]);
+    const double r = sc > 0 ? ad / sc : (ad > 0 ? INFINITY : 0.0);
+    mx = fmax(mx, r);
+  }
+  num = group_reduce<64>(num);
+  den = group_reduce<64>(den);
+  // max over the wave through the integer order of non-negative doubles
+  unsigned long long bx = (unsigned long long)__double_as_longlong(mx);
+  unsigned long long ba = (unsigned long long)__double_as_longlong(mabs);
+  if ((threadIdx.x & 63u) == 0) {
+    atomicAdd(&acc->num, num);
+    atomicAdd(&acc->den, den);
+  }
+  atomicMax(&acc->max_norm_bits, bx);
+  atomicMax(&acc->max_abs_bits, ba);
+}
+
+// ---- value conversion for low-precision storage --------------------------------------
+__global__ void k_to_lp(const void *in, int din, uint16_t *__restrict__ out, int dout, uint64_t n)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    // values are rounded to FP32 first, then to the 16-bit type (nearest-even)
+    const float f = din == MFP_F32 ? ((const float *)in)[i] : (float)((const double *)in)[i];
+    uint16_t b;
+    if (dout == MFP_BF16) {
+      __hip_bfloat16 h = __float2bfloat16(f);
+      b = *reinterpret_cast<uint16_t *>(&h);
+    } else {
+      __half h = __float2half(f);
+      b = __half_as_ushort(h);
+    }
+    out[i] = b;
+  }
+}
+
+// ---- low-precision CSR-vector (G lanes per row, Dot2 per lane) --------------------------
+template <int DT, int G>
+__global__ __launch_bounds__(256) void k_csr_vector_lp(uint32_t rows, const uint32_t *__restrict__ rp,
+                                                       const uint32_t *__restrict__ ci,
+                                                       const uint16_t *__restrict__ a,
+                                                       const float *__restrict__ x, float *__restrict__ y)
+{
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t row = (blk * 256u + threadIdx.x) / G;
+  const uint32_t lane = threadIdx.x % G;
+  comp acc{0.f, 0.f};
+  if (row < rows) {
+    const uint32_t b = rp[row], e = rp[row + 1];
+    uint32_t i = b + lane;
+    for (; i + 3u * G < e; i += 4u * G) {
+      const uint16_t a0 = ldg_stream(a + i), a1 = ldg_stream(a + i + G);
+      const uint16_t a2 = ldg_stream(a + i + 2 * G), a3 = ldg_stream(a + i + 3 * G);
+      const uint32_t c0 = ldg_stream(ci + i), c1 = ldg_stream(ci + i + G);
+      const uint32_t c2 = ldg_stream(ci + i + 2 * G), c3 = ldg_stream(ci + i + 3 * G);
+      const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+      dot2_add(acc, lp_val<DT>(a0), x0);
+      dot2_add(acc, lp_val<DT>(a1), x1);
+      dot2_add(acc, lp_val<DT>(a2), x2);
+      dot2_add(acc, lp_val<DT>(a3), x3);
+    }
+    for (; i < e; i += G) dot2_add(acc, lp_val<DT>(ldg_stream(a + i)), x[ldg_stream(ci + i)]);
+  }
+  acc = comp_group_reduce<G>(acc);
+  if (lane == 0 && row < rows) y[row] = acc.s + acc.c;
+}
+
+// ---- low-precision ELL (thread per row, Dot2, optional 16-bit column offsets) -------------
+template <int DT, bool OFF16>
+__global__ __launch_bounds__(256) void k_ell_lp(uint32_t rows, uint32_t w, const void *__restrict__ cols,
+                                                const uint32_t *__restrict__ base,
+                                                const uint16_t *__restrict__ ev,
+                                                const float *__restrict__ x, float *__restrict__ y)
+{
+  const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+  if (r >= rows) return;
+  comp acc{0.f, 0.f};
+  const uint32_t b0 = OFF16 ? base[r] : 0u;
+  uint64_t o = r;
+  uint32_t k = 0;
+  auto col = [&](uint64_t idx) -> uint32_t {
+    if constexpr (OFF16)
+      return b0 + (uint32_t)ldg_stream((const uint16_t *)cols + idx);
+    else
+      return ldg_stream((const uint32_t *)cols + idx);
+  };
+  for (; k + 4 <= w; k += 4) {
+    const uint16_t a0 = ldg_stream(ev + o), a1 = ldg_stream(ev + o + rows);
+    const uint16_t a2 = ldg_stream(ev + o + 2ull * rows), a3 = ldg_stream(ev + o + 3ull * rows);
+    const uint32_t c0 = col(o), c1 = col(o + rows), c2 = col(o + 2ull * rows), c3 = col(o + 3ull * rows);
+    const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    dot2_add(acc, lp_val<DT>(a0), x0);
+    dot2_add(acc, lp_val<DT>(a1), x1);
+    dot2_add(acc, lp_val<DT>(a2), x2);
+    dot2_add(acc, lp_val<DT>(a3), x3);
+    o += 4ull * rows;
+  }
+  for (; k < w; k++, o += rows) dot2_add(acc, lp_val<DT>(ldg_stream(ev + o)), x[col(o)]);
+  y[r] = acc.s + acc.c;
+}
+
+// Row column span (last - first) maximum, for choosing 16-bit offsets.
+__global__ void k_row_span(uint32_t rows, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                           uint32_t *mx)
+{
+  uint32_t m = 0;
+  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; r < rows; r += stride) {
+    const uint32_t b = rp[r], e = rp[r + 1];
+    if (e > b) m = max(m, ci[e - 1] - ci[b]);
+  }
+  atomicMax(mx, m);
+}
+
+// ELL build for 16-bit values: column-major, padding (0, offset 0 / col 0).
+__global__ void k_ell_lp_build(uint32_t rows, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                               const uint16_t *__restrict__ a, uint32_t w, bool off16,
+                               uint16_t *__restrict__ ev, void *cols, uint32_t *__restrict__ base)
+{
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const uint32_t s = rp[r], len = rp[r + 1] - s;
+  const uint32_t b0 = len ? ci[s] : 0u;
+  if (off16) base[r] = b0;
+  for (uint32_t k = 0; k < w; k++) {
+    const uint64_t o = (uint64_t)r + (uint64_t)k * rows;
+    const bool in = k < len;
+    ev[o] = in ? a[s + k] : (uint16_t)0;
+    const uint32_t c = in ? ci[s + k] : 0u;
+    if (off16)
+      ((uint16_t *)cols)[o] = in ? (uint16_t)(c - b0) : (uint16_t)0;
+    else
+      ((uint32_t *)cols)[o] = c;
+  }
+}
+
+static int launched_lp() { return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP); }
+
+int launch_spmv_f64(uint32_t rows, const uint32_t *rp, const uint32_t *ci, const void *vals, int vdt,
+                    const void *x, int xdt, double *y64, double *absdot, hipStream_t s)
+{
+  if (rows == 0) return MFP_OK;
+  k_spmv_f64<<<ceil_div_u32(rows, 4), 256, 0, s>>>(rows, rp, ci, vals, vdt, x, xdt, y64, absdot);
+  return launched_lp();
+}
+
+int launch_csr_vector_lp(int dt, uint32_t lanes, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
+                         const void *a, const void *x, void *y, hipStream_t s)
+{
+  if (rows == 0) return MFP_OK;
+  const uint32_t grid = ceil_div_u32((uint64_t)rows * lanes, 256);
+  const uint16_t *av = (const uint16_t *)a;
+  const float *xv = (const float *)x;
+  float *yv = (float *)y;
+#define LPV(D, G) k_csr_vector_lp<D, G><<<grid, 256, 0, s>>>(rows, rp, ci, av, xv, yv)
+  if (dt == MFP_BF16) {
+    switch (lanes) {
+    case 2: LPV(MFP_BF16, 2); break;
+    case 4: LPV(MFP_BF16, 4); break;
+    case 8: LPV(MFP_BF16, 8); break;
+    case 16: LPV(MFP_BF16, 16); break;
+    case 32: LPV(MFP_BF16, 32); break;
+    case 64: LPV(MFP_BF16, 64); break;
+    default: set_error("lanes"); return MFP_ERR_ARG;
+    }
+  } else {
+    switch (lanes) {
+    case 2: LPV(MFP_F16, 2); break;
+    case 4: LPV(MFP_F16, 4); break;
+    case 8: LPV(MFP_F16, 8); break;
+    case 16: LPV(MFP_F16, 16); break;
+    case 32: LPV(MFP_F16, 32); break;
+    case 64: LPV(MFP_F16, 64); break;
+    default: set_error("lanes"); return MFP_ERR_ARG;
+    }
+  }
+#undef LPV
+  return launched_lp();
+}
+
+int launch_ell_lp(int dt, bool off16, uint32_t rows, uint32_t w, const void *cols, const uint32_t *base,
+                  const void *ev, const void *x, void *y, hipStream_t s)
+{
+  if (rows == 0) return MFP_OK;
+  const uint32_t grid = ceil_div_u32(rows, 256);
+  const uint16_t *e = (const uint16_t *)ev;
+  const float *xv = (const float *)x;
+  float *yv = (float *)y;
+  if (dt == MFP_BF16) {
+    if (off16) k_ell_lp<MFP_BF16, true><<<grid, 256, 0, s>>>(rows, w, cols, base, e, xv, yv);
+    else k_ell_lp<MFP_BF16, false><<<grid, 256, 0, s>>>(rows, w, cols, base, e, xv, yv);
+  } else {
+    if (off16) k_ell_lp<MFP_F16, true><<<grid, 256, 0, s>>>(rows, w, cols, base, e, xv, yv);
+    else k_ell_lp<MFP_F16, false><<<grid, 256, 0, s>>>(rows, w, cols, base, e, xv, yv);
+  }
+  return launched_lp();
+}
+
+int row_span_max(uint32_t rows, const uint32_t *rp, const uint32_t *ci, uint32_t *out, hipStream_t s)
+{
+  uint32_t *d = nullptr;
+  MFP_CHECK(dmalloc((void **)&d, 4));
+  MFP_CHECK(fill_u32(d, 0, 1, s));
+  if (rows) k_row_span<<<std::min<uint32_t>(ceil_div_u32(rows, 256), 4096), 256, 0, s>>>(rows, rp, ci, d);
+  MFP_HIP(hipMemcpyAsync(out, d, 4, hipMemcpyDeviceToHost, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  dfree(d);
+  return MFP_OK;
+}
+
+int ell_lp_build(uint32_t rows, const uint32_t *rp, const uint32_t *ci, const void *a, uint32_t w,
+                 bool off16, void *ev, void *cols, uint32_t *base, hipStream_t s)
+{
+  if (rows == 0 || w == 0) return MFP_OK;
+  k_ell_lp_build<<<ceil_div_u32(rows, 256), 256, 0, s>>>(rows, rp, ci, (const uint16_t *)a, w, off16,
+                                                         (uint16_t *)ev, cols, base);
+  return launched_lp();
+}
+
+int to_lp_values(const void *in, int din, void *out, int dout, uint64_t n, hipStream_t s)
+{
+  if (n == 0) return MFP_OK;
+  k_to_lp<<<std::min<uint32_t>(ceil_div_u32(n, 256), 65536), 256, 0, s>>>(in, din, (uint16_t *)out, dout, n);
+  return launched_lp();
+}
+
+} // namespace mfp
+
+using namespace mfp;
+
+extern "C" {
+
+int mfp_spmv_f64_oracle(mfp_matrix m, int x_dtype, const void *x, double *y64, double *absdot,
+                        mfp_stream sv)
+{
+  MFP_ARG(m && x && y64, "null pointer");
+  MFP_ARG(x_dtype == MFP_F32 || x_dtype == MFP_F64, "x dtype must be f32 or f64");
+  mfp_matrix_view v;
+  MFP_CHECK(mfp_matrix_view_get(m, &v));
+  return launch_spmv_f64(v.rows, v.row_ptr, v.columns, v.values, v.dtype, x, x_dtype, y64, absdot,
+                         as_stream(sv));
+}
+
+int mfp_residual(uint32_t n, int y_dtype, const void *y, const double *y64, const double *absdot,
+                 mfp_error_report *out)
+{
+  MFP_ARG(out && y && y64, "null pointer");
+  MFP_ARG(y_dtype == MFP_F32 || y_dtype == MFP_F64, "y dtype must be f32 or f64");
+  MFP_CHECK(ensure_init());
+  resid_acc *acc = nullptr;
+  MFP_CHECK(dmalloc((void **)&acc, sizeof(resid_acc)));
+  MFP_CHECK(fill_bytes_zero(acc, sizeof(resid_acc), nullptr));
+  if (n) {
+    k_residual<<<std::min<uint32_t>(ceil_div_u32(n, 256), 2048), 256>>>(n, y, y_dtype, y64, absdot, acc);
+    MFP_HIP(hipGetLastError());
+  }
+  resid_acc h{};
+  MFP_HIP(hipMemcpy(&h, acc, sizeof(h), hipMemcpyDeviceToHost));
+  dfree(acc);
+  out->rel_l2 = h.den > 0 ? std::sqrt(h.num / h.den) : std::sqrt(h.num);
+  out->max_row_err = __builtin_bit_cast(double, h.max_norm_bits);
+  out->max_abs_err = __builtin_bit_cast(double, h.max_abs_bits);
+  return MFP_OK;
+}
+
+} // extern "C"

@@ -174,27 +174,35 @@ __device__ __forceinline__ T wg_row_sum(const uint32_t *__restrict__ ci, const T
   return r;
 }
 
+template <typename T>
+struct adaptive_smem {
+  T *cache;     // staged products (nnz_per_wg)
+  uint32_t *rs; // block-relative row starts (nnz_per_wg + 2)
+  T *wsum, *sv;
+  uint32_t *sf;
+};
+
+// One work item of the launch: hw is the item's dispatch index among grid
+// items (pieces of long rows first, then row blocks in XCD order).
 template <typename T, uint32_t K>
-__global__ __launch_bounds__(AD_T) void k_csr_adaptive(
-    uint32_t rows, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+__device__ __forceinline__ void adaptive_item(
+    uint32_t hw, uint32_t grid, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
     const T *__restrict__ a, const T *__restrict__ x, T *__restrict__ y,
-    const uint32_t *__restrict__ blocks, uint32_t nblocks, uint32_t nnz_per_wg, uint32_t long_row,
+    const uint32_t *__restrict__ blocks, uint32_t nnz_per_wg, uint32_t long_row,
     const uint4 *__restrict__ chunks, uint32_t nchunks, uint32_t nchunks_pad, uint32_t chunk,
-    T *__restrict__ partials)
+    T *__restrict__ partials, const adaptive_smem<T> &sm)
 {
   using VT = vec16<T>;
   constexpr uint32_t N = VT::N;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *cache = reinterpret_cast<T *>(smem_raw);
-  uint32_t *rs = reinterpret_cast<uint32_t *>(smem_raw + ((nnz_per_wg * sizeof(T) + 15u) & ~15u));
-  __shared__ T wsum[4];
-  __shared__ T sv[AD_T];
-  __shared__ uint32_t sf[AD_T];
+  T *cache = sm.cache;
+  uint32_t *rs = sm.rs;
+  T *wsum = sm.wsum, *sv = sm.sv;
+  uint32_t *sf = sm.sf;
   const uint32_t tid = threadIdx.x;
 
-  if (blockIdx.x < nchunks_pad) {
-    if (blockIdx.x >= nchunks) return;
-    const uint4 c4 = chunks[blockIdx.x]; // {row, piece, first item, pieces}
+  if (hw < nchunks_pad) {
+    if (hw >= nchunks) return;
+    const uint4 c4 = chunks[hw]; // {row, piece, first item, pieces}
     const uint32_t s = rp[c4.x], e = rp[c4.x + 1];
     const uint32_t cb = s + c4.y * chunk;
     const uint32_t ce = min(e, cb + chunk);
@@ -203,7 +211,7 @@ __global__ __launch_bounds__(AD_T) void k_csr_adaptive(
     return;
   }
 
-  const uint32_t item = xcd_block(blockIdx.x - nchunks_pad, gridDim.x - nchunks_pad);
+  const uint32_t item = xcd_block(hw - nchunks_pad, grid - nchunks_pad);
   const uint32_t r0 = blocks[item], r1 = blocks[item + 1];
   if (r1 <= r0) return;
   if (r1 - r0 == 1) {
@@ -328,6 +336,45 @@ __global__ __launch_bounds__(AD_T) void k_csr_adaptive(
     __syncthreads();
   }
   if (frow != 0xFFFFFFFFu) y[r0 + frow] = sv[tid - 1] + fval;
+}
+
+#define ADAPTIVE_SMEM(T)                                                                       \
+  extern __shared__ __align__(16) unsigned char smem_raw[];                                    \
+  __shared__ T wsum_[4];                                                                       \
+  __shared__ T sv_[AD_T];                                                                      \
+  __shared__ uint32_t sf_[AD_T];                                                               \
+  adaptive_smem<T> sm{reinterpret_cast<T *>(smem_raw),                                         \
+                      reinterpret_cast<uint32_t *>(smem_raw + ((nnz_per_wg * sizeof(T) + 15u) & ~15u)), \
+                      wsum_, sv_, sf_};
+
+template <typename T, uint32_t K>
+__global__ __launch_bounds__(AD_T) void k_csr_adaptive(
+    const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci, const T *__restrict__ a,
+    const T *__restrict__ x, T *__restrict__ y, const uint32_t *__restrict__ blocks,
+    uint32_t nnz_per_wg, uint32_t long_row, const uint4 *__restrict__ chunks, uint32_t nchunks,
+    uint32_t nchunks_pad, uint32_t chunk, T *__restrict__ partials)
+{
+  ADAPTIVE_SMEM(T)
+  adaptive_item<T, K>(blockIdx.x, gridDim.x, rp, ci, a, x, y, blocks, nnz_per_wg, long_row, chunks,
+                      nchunks, nchunks_pad, chunk, partials, sm);
+}
+
+// Persistent form: a fixed grid (a multiple of 8, so a workgroup keeps its
+// XCD) walks the items; used for batched launches.
+template <typename T, uint32_t K>
+__global__ __launch_bounds__(AD_T) void k_csr_adaptive_persistent(
+    uint32_t items, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+    const T *__restrict__ a, const T *__restrict__ x, T *__restrict__ y,
+    const uint32_t *__restrict__ blocks, uint32_t nnz_per_wg, uint32_t long_row,
+    const uint4 *__restrict__ chunks, uint32_t nchunks, uint32_t nchunks_pad, uint32_t chunk,
+    T *__restrict__ partials)
+{
+  ADAPTIVE_SMEM(T)
+  for (uint32_t hw = blockIdx.x; hw < items; hw += gridDim.x) {
+    adaptive_item<T, K>(hw, items, rp, ci, a, x, y, blocks, nnz_per_wg, long_row, chunks, nchunks,
+                        nchunks_pad, chunk, partials, sm);
+    __syncthreads();
+  }
 }
 
 // One lane per long row: add its pieces left to right.
@@ -572,12 +619,21 @@ uint32_t adaptive_vectors_per_lane(int dtype, uint32_t W)
 }
 
 template <typename T, uint32_t K>
-static void adaptive_launch_k(const adaptive_args &g, const T *a, const T *x, T *y, hipStream_t s,
-                              size_t lds, uint32_t grid, uint32_t nchunks_pad)
+static int adaptive_launch_k(const adaptive_args &g, const T *a, const T *x, T *y, hipStream_t s,
+                             size_t lds, uint32_t items, uint32_t nchunks_pad, uint32_t persistent_grid)
 {
-  k_csr_adaptive<T, K><<<grid, AD_T, lds, s>>>(g.rows, g.rp, g.ci, a, x, y, g.blocks, g.nblocks,
-                                               g.nnz_per_wg, g.long_row, g.chunks, g.nchunks,
-                                               nchunks_pad, g.chunk, (T *)g.partials);
+  const void *fn = persistent_grid ? (const void *)k_csr_adaptive_persistent<T, K>
+                                   : (const void *)k_csr_adaptive<T, K>;
+  if (lds > 65536) MFP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (persistent_grid)
+    k_csr_adaptive_persistent<T, K><<<persistent_grid, AD_T, lds, s>>>(
+        items, g.rp, g.ci, a, x, y, g.blocks, g.nnz_per_wg, g.long_row, g.chunks, g.nchunks,
+        nchunks_pad, g.chunk, (T *)g.partials);
+  else
+    k_csr_adaptive<T, K><<<items, AD_T, lds, s>>>(g.rp, g.ci, a, x, y, g.blocks, g.nnz_per_wg,
+                                                  g.long_row, g.chunks, g.nchunks, nchunks_pad,
+                                                  g.chunk, (T *)g.partials);
+  return MFP_OK;
 }
 
 // Dynamic LDS of k_csr_adaptive: the staged products, then the row starts.
@@ -591,27 +647,27 @@ static int adaptive_t(const adaptive_args &g, const T *a, const T *x, T *y, hipS
 {
   // chunk items first; padded to a multiple of 8 so block ids keep their XCD
   const uint32_t nchunks_pad = (g.nchunks + 7u) & ~7u;
-  const uint32_t grid = g.nblocks ? g.nblocks + nchunks_pad : nchunks_pad;
+  const uint32_t items = g.nblocks ? g.nblocks + nchunks_pad : nchunks_pad;
   const size_t lds = adaptive_lds_bytes(sizeof(T) == 4 ? MFP_F32 : MFP_F64, g.nnz_per_wg);
   const uint32_t K = adaptive_vectors_per_lane(sizeof(T) == 4 ? MFP_F32 : MFP_F64, g.nnz_per_wg);
-  const void *fn = nullptr;
-  switch (K) {
-  case 1: fn = (const void *)k_csr_adaptive<T, 1>; break;
-  case 2: fn = (const void *)k_csr_adaptive<T, 2>; break;
-  case 4: fn = (const void *)k_csr_adaptive<T, 4>; break;
-  case 8: fn = (const void *)k_csr_adaptive<T, 8>; break;
-  case 16: fn = (const void *)k_csr_adaptive<T, 16>; break;
-  default: set_error("nnz_per_wg too large"); return MFP_ERR_ARG;
+  // persistent: 8 workgroups per CU (the LDS/wave occupancy), a multiple of 8
+  uint32_t pgrid = 0;
+  if (g.persistent) {
+    pgrid = std::max<uint32_t>(8, (uint32_t)std::max(1, props().cu_count) * 8u);
+    pgrid = std::min<uint32_t>(pgrid, (items + 7u) & ~7u);
+    pgrid = std::max<uint32_t>(8, pgrid & ~7u);
   }
-  if (lds > 65536) MFP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  if (grid) {
+  if (items) {
+    int st;
     switch (K) {
-    case 1: adaptive_launch_k<T, 1>(g, a, x, y, s, lds, grid, nchunks_pad); break;
-    case 2: adaptive_launch_k<T, 2>(g, a, x, y, s, lds, grid, nchunks_pad); break;
-    case 4: adaptive_launch_k<T, 4>(g, a, x, y, s, lds, grid, nchunks_pad); break;
-    case 8: adaptive_launch_k<T, 8>(g, a, x, y, s, lds, grid, nchunks_pad); break;
-    default: adaptive_launch_k<T, 16>(g, a, x, y, s, lds, grid, nchunks_pad); break;
+    case 1: st = adaptive_launch_k<T, 1>(g, a, x, y, s, lds, items, nchunks_pad, pgrid); break;
+    case 2: st = adaptive_launch_k<T, 2>(g, a, x, y, s, lds, items, nchunks_pad, pgrid); break;
+    case 4: st = adaptive_launch_k<T, 4>(g, a, x, y, s, lds, items, nchunks_pad, pgrid); break;
+    case 8: st = adaptive_launch_k<T, 8>(g, a, x, y, s, lds, items, nchunks_pad, pgrid); break;
+    case 16: st = adaptive_launch_k<T, 16>(g, a, x, y, s, lds, items, nchunks_pad, pgrid); break;
+    default: set_error("nnz_per_wg too large"); return MFP_ERR_ARG;
     }
+    MFP_CHECK(st);
     MFP_CHECK(launched());
   }
   if (g.nchunks) {

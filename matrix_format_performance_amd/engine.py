@@ -129,16 +129,43 @@ def download(ptr: int, n: int, dtype) -> np.ndarray:
     return out
 
 
-class DeviceMatrix:
-    """CSR matrix in HBM (u32 row_ptr/columns, f32 or f64 values)."""
+VALUE_TYPES = {"f32": L.MFP_F32, "f64": L.MFP_F64, "bf16": L.MFP_BF16, "f16": L.MFP_F16}
+_VT_NAME = {v: k for k, v in VALUE_TYPES.items()}
 
-    def __init__(self, handle, dtype):
+
+class DeviceMatrix:
+    """CSR matrix in HBM (u32 row_ptr/columns; f32, f64, bf16 or f16 values).
+
+    ``dtype`` is the numpy type of the values (uint16 bit patterns for bf16 /
+    f16) and ``value_type`` its name."""
+
+    def __init__(self, handle, dtype=None):
         self.handle = handle
-        self.dtype = np.dtype(dtype)
         v = L.matrix_view()
         L.call("mfp_matrix_view_get", handle, C.byref(v))
+        self.value_type = _VT_NAME[v.dtype]
+        self.dtype = {"f32": np.dtype(np.float32), "f64": np.dtype(np.float64)}.get(
+            self.value_type, np.dtype(np.uint16))
         self.rows, self.cols, self.nnz = v.rows, v.cols, v.nnz
         self.row_ptr_ptr, self.columns_ptr, self.values_ptr = v.row_ptr, v.columns, v.values
+
+    @property
+    def vector_dtype(self):
+        """x / y type: the value type, FP32 for 16-bit values."""
+        return np.dtype(np.float64) if self.value_type == "f64" else np.dtype(np.float32)
+
+    def convert(self, value_type: str) -> "DeviceMatrix":
+        """Copy with values converted (16-bit types rounded from FP32)."""
+        h = C.c_void_p()
+        L.call("mfp_matrix_convert", self.handle, VALUE_TYPES[value_type], C.byref(h))
+        return DeviceMatrix(h)
+
+    def oracle_f64(self, x: "DeviceArray", stream=None):
+        """FP64 SpMV on the GPU -> (y64, absdot) device arrays."""
+        y64 = DeviceArray(self.rows, np.float64)
+        ad = DeviceArray(self.rows, np.float64)
+        L.call("mfp_spmv_f64_oracle", self.handle, dtype_code(x.dtype), x.ptr, y64.ptr, ad.ptr, stream)
+        return y64, ad
 
     @classmethod
     def from_csr(cls, row_ptr, columns, values, cols=None) -> "DeviceMatrix":
@@ -189,6 +216,14 @@ class DeviceMatrix:
             pass
 
 
+def residual(y: "DeviceArray", y64: "DeviceArray", absdot: "DeviceArray" = None) -> dict:
+    """Error of a device result against the FP64 oracle (computed on the GPU)."""
+    r = L.error_report()
+    L.call("mfp_residual", y.n, dtype_code(y.dtype), y.ptr, y64.ptr, absdot.ptr if absdot else None,
+           C.byref(r))
+    return {"rel_l2": r.rel_l2, "max_row_err": r.max_row_err, "max_abs_err": r.max_abs_err}
+
+
 def default_tuning(matrix: DeviceMatrix, fmt: str) -> L.tuning:
     t = L.tuning()
     L.call("mfp_tuning_default", matrix.handle, FORMATS[fmt], C.byref(t))
@@ -213,6 +248,10 @@ class Plan:
     def spmv(self, x: DeviceArray, y: DeviceArray, stream=None) -> None:
         L.call("mfp_plan_spmv", self.handle, x.ptr, y.ptr, stream)
 
+    def spmm(self, k: int, X: DeviceArray, Y: DeviceArray, stream=None) -> None:
+        """Y (rows x k) = A X (cols x k), row-major."""
+        L.call("mfp_plan_spmm", self.handle, k, X.ptr, Y.ptr, stream)
+
     def info(self) -> dict:
         i = L.plan_info()
         L.call("mfp_plan_get_info", self.handle, C.byref(i))
@@ -227,6 +266,102 @@ class Plan:
     def destroy(self):
         if self.handle:
             L.load().mfp_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _tuning_dict(t: L.tuning) -> dict:
+    return {f: getattr(t, f) for f, _ in L.tuning._fields_}
+
+
+def autotune(matrix: DeviceMatrix, x: DeviceArray, y: DeviceArray, formats=None, reps: int = 3,
+             flush: bool = True):
+    """Time every candidate (format, tuning) the library sweeps for this
+    matrix; returns a list of {format, tuning, ms, status} sorted by time."""
+    names = list(FORMATS) if formats is None else list(formats)
+    mask = 0
+    for f in names:
+        mask |= 1 << FORMATS[f]
+    n = C.c_uint32()
+    L.call("mfp_autotune_candidates", matrix.handle, mask, None, 0, C.byref(n))
+    arr = (L.candidate * n.value)()
+    L.call("mfp_autotune_candidates", matrix.handle, mask, arr, n.value, C.byref(n))
+    L.call("mfp_autotune_run", matrix.handle, x.ptr, y.ptr, arr, n.value, reps, int(flush))
+    out = [{"format": FORMAT_NAMES[c.format], "tuning": _tuning_dict(c.tuning), "ms": c.ms,
+            "status": c.status} for c in arr]
+    return sorted(out, key=lambda c: c["ms"])
+
+
+class Graph:
+    """An instantiated HIP graph of a batched launch."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream=None) -> None:
+        L.call("mfp_graph_launch", self.handle, stream)
+
+    def destroy(self):
+        if self.handle:
+            L.load().mfp_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many matrices assembled into one block-diagonal CSR, multiplied by one
+    persistent launch (CSR-Adaptive for k = 1, CSR-vector SpMM for k > 1)."""
+
+    def __init__(self, matrices, lanes: int = 8, stream=None, **adaptive_tuning):
+        arr = (C.c_void_p * len(matrices))(*[m.handle for m in matrices])
+        t = None
+        if adaptive_tuning:
+            t = default_tuning(matrices[0], "csr_adaptive")
+            for k, v in adaptive_tuning.items():
+                setattr(t, k, v)
+        h = C.c_void_p()
+        L.call("mfp_batch_create", C.byref(h), arr, len(matrices), lanes, C.byref(t) if t else None, stream)
+        self.handle = h
+        self.count = len(matrices)
+        r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        self.row_offsets = np.zeros(self.count + 1, np.uint32)
+        self.col_offsets = np.zeros(self.count + 1, np.uint32)
+        L.call("mfp_batch_info", h, C.byref(r), C.byref(c), C.byref(n), _ptr(self.row_offsets),
+               _ptr(self.col_offsets))
+        self.rows, self.cols, self.nnz = r.value, c.value, n.value
+        self.dtype = matrices[0].dtype
+
+    def matrix_arrays(self):
+        """Download the assembled block-diagonal CSR (row_ptr, columns, values)."""
+        m = C.c_void_p()
+        L.call("mfp_batch_matrix", self.handle, C.byref(m))
+        rp = np.empty(self.rows + 1, np.uint32)
+        ci = np.empty(self.nnz, np.uint32)
+        va = np.empty(self.nnz, self.dtype)
+        L.call("mfp_matrix_download", m, _ptr(rp), _ptr(ci), _ptr(va))
+        return rp, ci, va
+
+    def spmm(self, k: int, X: DeviceArray, Y: DeviceArray, stream=None) -> None:
+        L.call("mfp_batch_spmm", self.handle, k, X.ptr, Y.ptr, stream)
+
+    def graph(self, k: int, X: DeviceArray, Y: DeviceArray) -> Graph:
+        g = C.c_void_p()
+        L.call("mfp_batch_graph_create", self.handle, k, X.ptr, Y.ptr, C.byref(g))
+        return Graph(g)
+
+    def destroy(self):
+        if self.handle:
+            L.load().mfp_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -1,0 +1,135 @@
+"""SpMM (k right-hand sides), batched matrices in one persistent launch, and
+HIP-graph replay of that launch."""
+import numpy as np
+import pytest
+
+import oracle
+from gpu_cases import announce, case
+
+pytestmark = pytest.mark.gpu
+
+
+def _columns_ref(rp, ci, va, X, lanes):
+    return np.stack([oracle.spmv_lanes(rp, ci, va, np.ascontiguousarray(X[:, j]), lanes)
+                     for j in range(X.shape[1])], axis=1)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("lanes", [4, 32])
+@pytest.mark.parametrize("name", ["stencil3d_20", "powerlaw_20k"])
+def test_spmm_columns_bit_exact(gpu, name, k, lanes, dtype):
+    rp, ci, va, cols = case(name, dtype)
+    announce(f"spmm case={name} k={k} lanes={lanes} dtype={dtype}")
+    rng = np.random.default_rng(k)
+    X = rng.uniform(-1, 1, (cols, k)).astype(dtype)
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "csr_vector", lanes=lanes)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray((rp.size - 1) * k, np.dtype(dtype))
+    Yd.fill(np.nan)
+    plan.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Y = Yd.to_host().reshape(-1, k)
+    assert np.array_equal(Y, _columns_ref(rp, ci, va, X, lanes))
+
+
+def _batch_members(gpu, count, dtype, seed=0):
+    rng = np.random.default_rng(seed)
+    mats = []
+    for i in range(count):
+        rows = int(np.exp(rng.uniform(np.log(500), np.log(20000))))
+        fam = ["stencil2d", "stencil3d", "uniform", "powerlaw"][i % 4]
+        if fam == "stencil2d":
+            spec = gpu.GenSpec("stencil2d", values="uniform", n=max(2, int(np.sqrt(rows))), seed=i)
+        elif fam == "stencil3d":
+            spec = gpu.GenSpec("stencil3d", values="normal", n=max(2, int(round(rows ** (1 / 3)))), seed=i)
+        elif fam == "uniform":
+            spec = gpu.GenSpec("uniform", values="uniform", rows=rows, nnz_lo=4, nnz_hi=64, bandwidth=0.1, seed=i)
+        else:
+            spec = gpu.GenSpec("powerlaw", values="uniform", rows=rows, nnz_lo=1, nnz_hi=rows // 2,
+                               alpha=float(rng.uniform(1.5, 3.0)), seed=i)
+        mats.append(gpu.generate(spec, np.dtype(dtype)))
+    return mats
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("k", [1, 4, 16])
+def test_batch_persistent_launch(gpu, k, dtype):
+    announce(f"batch k={k} dtype={dtype}")
+    members = _batch_members(gpu, 24, dtype, seed=k)
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    batch = gpu.Batch(dms, lanes=8, long_row=512, chunk=256)
+    for d in dms:
+        d.destroy()  # the batch holds its own copy
+    assert batch.count == 24 and batch.rows == sum(m[0].size - 1 for m in members)
+    rp, ci, va = batch.matrix_arrays()
+    # the assembly is the block-diagonal concatenation
+    r_off, c_off = batch.row_offsets, batch.col_offsets
+    for j, (mrp, mci, mva) in enumerate(members):
+        e0 = rp[r_off[j]]
+        assert np.array_equal(rp[r_off[j]:r_off[j + 1] + 1] - e0, mrp)
+        assert np.array_equal(ci[e0:e0 + mci.size], mci + c_off[j])
+        assert np.array_equal(va[e0:e0 + mva.size], mva)
+    X = np.random.default_rng(3).uniform(-1, 1, (batch.cols, k)).astype(dtype)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray(batch.rows * k, np.dtype(dtype))
+    Yd.fill(np.nan)
+    batch.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Y = Yd.to_host().reshape(-1, k)
+    if k == 1:
+        W = gpu.engine.DEFAULT_NNZ_PER_WG[np.dtype(dtype)]
+        ref = oracle.spmv_adaptive(rp, ci, va, X[:, 0].copy(), W, 256, 512, 256)[:, None]
+    else:
+        ref = _columns_ref(rp, ci, va, X, 8)
+    assert np.array_equal(Y, ref)
+
+
+def test_batch_graph_replays_match_eager(gpu):
+    announce("batch graph replay")
+    members = _batch_members(gpu, 16, "float32", seed=7)
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    batch = gpu.Batch(dms, lanes=16, long_row=512, chunk=256)
+    for k in (1, 4):
+        Xd = gpu.DeviceArray(batch.cols * k, np.float32)
+        Yg = gpu.DeviceArray(batch.rows * k, np.float32)
+        Ye = gpu.DeviceArray(batch.rows * k, np.float32)
+        g = batch.graph(k, Xd, Yg)
+        for rep in range(4):
+            X = np.random.default_rng(rep).uniform(-1, 1, batch.cols * k).astype(np.float32)
+            gpu.engine.L.call("mfp_memcpy_h2d", Xd.ptr, X.ctypes.data, X.nbytes)
+            Yg.fill(np.nan)
+            g.launch()
+            gpu.synchronize()
+            batch.spmm(k, Xd, Ye)  # eager work between replays
+            gpu.synchronize()
+            assert np.array_equal(Yg.to_host(), Ye.to_host()), (k, rep)
+        g.destroy()
+
+
+def test_autotune_sweeps_and_times_candidates(gpu):
+    rp, ci, va, cols = case("powerlaw_20k", "float32")
+    announce("autotune powerlaw_20k")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    x = gpu.DeviceArray.from_host(gpu.generate_vector(1, cols, np.float32))
+    y = gpu.DeviceArray(rp.size - 1, np.float32)
+    res = gpu.autotune(A, x, y, reps=2)
+    fmts = [r["format"] for r in res]
+    assert fmts.count("csr_vector") == 6 and fmts.count("csr_adaptive") == 3
+    # hybrid splits: average and the 0.75 / 0.9 / 0.99 row-length quantiles
+    lens = np.sort(np.diff(rp.astype(np.int64)))
+    q = lambda p: int(lens[int(p * (lens.size - 1))])
+    widths = sorted({-(-ci.size // (rp.size - 1)), q(0.75), q(0.9), q(0.99)} - {0})
+    assert sorted(r["tuning"]["hyb_width"] for r in res if r["format"] == "hyb") == widths
+    ok = [r for r in res if r["status"] == 0]
+    assert ok and all(np.isfinite(r["ms"]) and r["ms"] > 0 for r in ok)
+    assert res[0]["ms"] == min(r["ms"] for r in res)
+    # ELL with width 12000 x 20k rows builds; a plan from the best candidate reproduces it
+    best = res[0]
+    plan = gpu.Plan(A, best["format"], **{k: v for k, v in best["tuning"].items()})
+    plan.spmv(x, y)
+    gpu.synchronize()
+    y_seq = oracle.spmv_seq(rp, ci, va, x.to_host())
+    _, absdot = oracle.spmv_f64(rp, ci, va, x.to_host())
+    assert oracle.tolerance_check(y.to_host(), y_seq, absdot, 1e-5) <= 1e-5
