@@ -164,7 +164,10 @@ typedef struct {
   uint64_t budget_bytes;  /* refuse layouts above this (0 = 90% of free HBM) */
   uint32_t flags;         /* MFP_TUNE_* */
 } mfp_tuning;
-enum { MFP_TUNE_NO_OFFSET16 = 1 /* 16-bit-value ELL: keep 32-bit columns */ };
+enum {
+  MFP_TUNE_NO_OFFSET16 = 1, /* 16-bit-value ELL: keep 32-bit columns */
+  MFP_TUNE_IS_DEFAULT = 2   /* set by the autotuner on each format's default candidate; no effect */
+};
 
 typedef struct mfp_plan_s *mfp_plan;
 int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *out);
@@ -228,9 +231,11 @@ int mfp_plan_destroy(mfp_plan p);
 /* ---- autotuner (no reference counterpart) -----------------------------------
  * Candidates sweep lanes per row (CSR-vector), the CSR-Adaptive block size,
  * the ELL/COO split of the hybrids (row-length quantiles) and the SCOO slice
- * size.  mfp_autotune_run builds and times each (median of reps event-timed
- * SpMVs, a cache flush before each when flush != 0); a candidate that cannot
- * be built keeps ms = +inf and its status. */
+ * size, plus each format's default tuning (marked MFP_TUNE_IS_DEFAULT; the
+ * hybrids' default takes its width from hyb_percent).  mfp_autotune_run
+ * builds and times each (median of reps event-timed SpMVs, a cache flush
+ * before each when flush != 0); a candidate that cannot be built keeps
+ * ms = +inf and its status. */
 typedef struct {
   int format;
   mfp_tuning tuning;
@@ -242,6 +247,31 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
                             uint32_t *n);
 int mfp_autotune_run(mfp_matrix m, const void *x, void *y, mfp_candidate *cands, uint32_t n,
                      uint32_t reps, int flush);
+
+/* ---- learned format selector (no reference counterpart) -------------------
+ * mfp_matrix_features writes 64 structure features of a matrix (row-length
+ * histograms and moments, distance from the diagonal, row spans; computed on
+ * the device from integer counters, so they do not depend on atomic order).
+ * The selector network is an MLP 64 -> 128 -> 128 -> 8 (ReLU, softmax over
+ * the eight formats, 25,864 FP32 parameters) whose forward, backward
+ * (softmax cross-entropy against target distributions) and Adam update run
+ * in this library's HIP kernels.  Parameters are packed
+ * W1[64][128] b1[128] W2[128][128] b2[128] W3[128][8] b3[8] (W[in][out]).
+ * Inference with bf16 != 0 rounds activations and weights to BF16 and
+ * multiplies on the matrix cores with FP32 accumulation.  X is n x 64 and
+ * P, probs are n x 8, row-major, on the host; loss is the batch mean. */
+int mfp_matrix_features(mfp_matrix m, float *out /* 64 */);
+typedef struct mfp_mlp_s *mfp_mlp;
+int mfp_mlp_create(mfp_mlp *out, uint64_t seed);
+int mfp_mlp_destroy(mfp_mlp m);
+uint32_t mfp_mlp_param_count(void);
+int mfp_mlp_get_params(mfp_mlp m, float *out);
+int mfp_mlp_set_params(mfp_mlp m, const float *in); /* also resets Adam state */
+int mfp_mlp_loss_grad(mfp_mlp m, const float *X, const float *P, uint32_t n, double *loss,
+                      float *grad);
+int mfp_mlp_train_step(mfp_mlp m, const float *X, const float *P, uint32_t n, float lr,
+                       double *loss);
+int mfp_mlp_predict(mfp_mlp m, const float *X, uint32_t n, int bf16, float *probs);
 
 /* ---- host converters run on the device (compat layer) --------------------- */
 /* fill_row_blocks (gpu/csr_adaptive_spmv.cu:180-240), nnz_per_wg generalising

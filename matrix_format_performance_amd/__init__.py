@@ -18,6 +18,7 @@ from .reference_api import (compare_results, coo_matrix_class, csr_matrix_class,
                             gpu_hybrid_spmv, gpu_scoo_spmv, hybrid_matrix_class, matrix_class,
                             matrix_meta, matrix_rows_statistic, measurement_class,
                             resizable_gpu_memory, scoo_geometry, scoo_matrix_class)
+from . import selector  # noqa: E402
 
 __all__ = [
     "MfpError", "MfpBudgetError", "FORMATS", "LABELS", "Batch", "autotune", "DeviceArray", "DeviceMatrix",
@@ -27,5 +28,5 @@ __all__ = [
     "gpu_csr_adaptive_spmv", "gpu_csr_spmv", "gpu_csr_vector_spmv", "gpu_ell_spmv",
     "gpu_hybrid_atomic_spmv", "gpu_hybrid_spmv", "gpu_scoo_spmv", "hybrid_matrix_class",
     "matrix_class", "matrix_meta", "matrix_rows_statistic", "measurement_class",
-    "resizable_gpu_memory", "scoo_geometry", "scoo_matrix_class",
+    "resizable_gpu_memory", "scoo_geometry", "scoo_matrix_class", "selector",
 ]

@@ -58,6 +58,7 @@ class tuning(C.Structure):
 
 
 MFP_TUNE_NO_OFFSET16 = 1
+MFP_TUNE_IS_DEFAULT = 2
 
 
 class candidate(C.Structure):
@@ -162,6 +163,15 @@ SIGNATURES = {
     "mfp_plan_destroy": (i32, [vp]),
     "mfp_autotune_candidates": (i32, [vp, u32, vp, u32, P(u32)]),
     "mfp_autotune_run": (i32, [vp, vp, vp, vp, u32, u32, i32]),
+    "mfp_matrix_features": (i32, [vp, vp]),
+    "mfp_mlp_create": (i32, [P(vp), u64]),
+    "mfp_mlp_destroy": (i32, [vp]),
+    "mfp_mlp_param_count": (u32, []),
+    "mfp_mlp_get_params": (i32, [vp, vp]),
+    "mfp_mlp_set_params": (i32, [vp, vp]),
+    "mfp_mlp_loss_grad": (i32, [vp, vp, vp, u32, P(dbl), vp]),
+    "mfp_mlp_train_step": (i32, [vp, vp, vp, u32, C.c_float, P(dbl)]),
+    "mfp_mlp_predict": (i32, [vp, vp, u32, i32, vp]),
     "mfp_fill_row_blocks": (i32, [u32, vp, u32, vp, P(u32)]),
     "mfp_scoo_geometry": (i32, [i32, u32, u32, u64, P(u32), P(u32), P(u32)]),
     "mfp_compare_results": (i32, [i32, u32, vp, vp]),

@@ -24,6 +24,8 @@ SOURCES = [
     "reference_api.hip",
     "precision.hip",
     "batch.hip",
+    "features.hip",
+    "mlp.hip",
     "autotune.cpp",
     "generate.cpp",
 ]

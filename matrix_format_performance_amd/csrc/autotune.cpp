@@ -67,34 +67,41 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
     mfp_tuning t;
     st = mfp_tuning_default(m, f, &t);
     if (st != MFP_OK) return st;
+    const mfp_tuning def = t;
+    mfp_tuning tagged = def;
+    tagged.flags |= MFP_TUNE_IS_DEFAULT;
     switch (f) {
     case MFP_CSR_VECTOR:
       for (uint32_t l = 2; l <= 64; l <<= 1) {
         t.lanes = l;
-        push(c, f, t);
+        push(c, f, l == def.lanes ? tagged : t);
       }
       break;
     case MFP_CSR_ADAPTIVE: {
       const uint32_t base = t.nnz_per_wg;
       for (uint32_t w : {base / 2, base, base * 2}) {
         t.nnz_per_wg = w;
-        push(c, f, t);
+        push(c, f, w == base ? tagged : t);
       }
       break;
     }
     case MFP_SCOO: {
+      push(c, f, tagged);
       const uint32_t acc = (v.dtype == MFP_F64 ? 4096u : 8192u);
       for (uint32_t ss : {256u, 512u, 1024u, 2048u, 4096u}) {
         if (v.rows && ss > std::max<uint32_t>(256, v.rows)) break;
         t.scoo_slice_size = ss;
         t.scoo_lanes = std::max<uint32_t>(1, std::min<uint32_t>(32, prev_pow2(acc / ss)));
         t.scoo_slices = v.rows ? (v.rows + ss - 1) / ss : 1;
+        if (ss == def.scoo_slice_size && t.scoo_lanes == def.scoo_lanes && t.scoo_slices == def.scoo_slices)
+          continue;
         push(c, f, t);
       }
       break;
     }
     case MFP_HYB:
     case MFP_HYB_ATOMIC: {
+      push(c, f, tagged); // width from the hyb_percent rule
       std::vector<uint32_t> ws = {avg, quant(0.75), quant(0.9), quant(0.99)};
       std::sort(ws.begin(), ws.end());
       ws.erase(std::unique(ws.begin(), ws.end()), ws.end());
@@ -105,7 +112,7 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
       }
       break;
     }
-    default: push(c, f, t); break;
+    default: push(c, f, tagged); break;
     }
   }
   *n = (uint32_t)c.size();

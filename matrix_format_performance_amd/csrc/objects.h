@@ -26,7 +26,7 @@ struct mfp_plan_s {
   uint32_t *coo_col = nullptr, *coo_row = nullptr;
   uint32_t slices = 0, ss = 0, L = 0, scoo_threads = 512;
   uint32_t *sp = nullptr;
-  adaptive_layout ad;
+  mfp::adaptive_layout ad;
   uint64_t device_bytes = 0;
   // 16-bit-value ELL: columns as u16 offsets from base[row] when off16
   bool off16 = false;

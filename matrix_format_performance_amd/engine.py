@@ -440,7 +440,7 @@ def generate_vector(seed: int, n: int, dtype=np.float32) -> np.ndarray:
     return x[:n]
 
 
-# BASELINE workloads (sizes of the benchmark configurations)
+# Benchmark workloads: 2D 5-point 256x256, power-law 1M rows, 3D 27-point 128^3
 def workload(name: str, dtype=np.float32, seed: int = 0):
     if name == "stencil2d_256":
         return generate(GenSpec("stencil2d", values="laplace", n=256, seed=seed), dtype)

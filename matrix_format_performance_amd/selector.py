@@ -1,0 +1,361 @@
+"""Learned format selector.
+
+A matrix is described by 64 structure features (``features``, computed on the
+device by ``mfp_matrix_features``); an MLP 64 -> 128 -> 128 -> 8 maps them to
+a distribution over the eight formats.  The network is trained on the GPU by
+the library's own HIP kernels (forward, backward, Adam) against targets made
+from timings this package measures (``sweep``: every autotuner candidate of
+every format, per matrix), and it infers in FP32 or in BF16 on the matrix
+cores.  The pick is the most probable format with its default tuning; the
+exhaustive autotuner stays the fallback and the judge of the pick's regret.
+
+Command line (``python -m matrix_format_performance_amd.selector``):
+
+* ``sweep --out S.jsonl --count N [--max-nnz M] [--seed K]``: generates
+  matrices, records features and candidate timings, one JSON line per matrix;
+  rerunning with the same file resumes after the last complete line.
+* ``train --data S.jsonl --out W.npy [--train 1600]``: trains on the first
+  ``--train`` records and reports regret on the rest.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import engine as E
+
+NI, NH, NO = 64, 128, 8
+FORMAT_ORDER = [E.FORMAT_NAMES[i] for i in range(NO)]
+
+
+def features(matrix: "E.DeviceMatrix") -> np.ndarray:
+    """The 64 structure features of a device matrix (float32)."""
+    out = np.empty(NI, np.float32)
+    L.call("mfp_matrix_features", matrix.handle, out.ctypes.data)
+    return out
+
+
+class MLP:
+    """The selector network; parameters live on the device."""
+
+    def __init__(self, seed: int = 0):
+        E.lib()
+        h = C.c_void_p()
+        L.call("mfp_mlp_create", C.byref(h), seed)
+        self.handle = h.value
+        self.nparam = L.load().mfp_mlp_param_count()
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        L.call("mfp_mlp_get_params", self.handle, out.ctypes.data)
+        return out
+
+    @params.setter
+    def params(self, theta) -> None:
+        theta = np.ascontiguousarray(theta, np.float32)
+        if theta.size != self.nparam:
+            raise ValueError(f"expected {self.nparam} parameters, got {theta.size}")
+        L.call("mfp_mlp_set_params", self.handle, theta.ctypes.data)
+
+    @staticmethod
+    def _xp(X, P=None):
+        X = np.ascontiguousarray(X, np.float32)
+        if X.ndim != 2 or X.shape[1] != NI or X.shape[0] == 0:
+            raise ValueError(f"X must be n x {NI} with n > 0")
+        if P is not None:
+            P = np.ascontiguousarray(P, np.float32)
+            if P.shape != (X.shape[0], NO):
+                raise ValueError(f"P must be {X.shape[0]} x {NO}")
+        return X, P
+
+    def loss_grad(self, X, P):
+        X, P = self._xp(X, P)
+        loss = C.c_double()
+        g = np.empty(self.nparam, np.float32)
+        L.call("mfp_mlp_loss_grad", self.handle, X.ctypes.data, P.ctypes.data, X.shape[0], C.byref(loss),
+               g.ctypes.data)
+        return loss.value, g
+
+    def train_step(self, X, P, lr: float) -> float:
+        X, P = self._xp(X, P)
+        loss = C.c_double()
+        L.call("mfp_mlp_train_step", self.handle, X.ctypes.data, P.ctypes.data, X.shape[0], lr, C.byref(loss))
+        return loss.value
+
+    def predict(self, X, bf16: bool = False) -> np.ndarray:
+        X, _ = self._xp(X)
+        out = np.empty((X.shape[0], NO), np.float32)
+        L.call("mfp_mlp_predict", self.handle, X.ctypes.data, X.shape[0], int(bool(bf16)), out.ctypes.data)
+        return out
+
+    def destroy(self):
+        if getattr(self, "handle", None):
+            L.load().mfp_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ---- targets and regret -------------------------------------------------------
+
+def format_times(record: dict, which: str = "best") -> np.ndarray:
+    """Per-format time (ms) of a sweep record: ``best`` over the format's
+    candidates or its ``default`` candidate; +inf where none could be built."""
+    key = "best_ms" if which == "best" else "default_ms"
+    return np.array([record[key].get(f, math.inf) for f in FORMAT_ORDER], np.float64)
+
+
+def soft_targets(times: np.ndarray, tau: float = 0.05) -> np.ndarray:
+    """Target distribution over formats: p_f proportional to
+    exp(-(t_f / t_best - 1) / tau), zero for formats that could not run."""
+    times = np.atleast_2d(times)
+    best = np.min(times, axis=1, keepdims=True)
+    z = -(times / best - 1.0) / tau
+    z[~np.isfinite(times)] = -np.inf
+    z -= z.max(axis=1, keepdims=True)
+    p = np.exp(z)
+    return (p / p.sum(axis=1, keepdims=True)).astype(np.float32)
+
+
+def regret(times_pick: np.ndarray, times_best: np.ndarray) -> np.ndarray:
+    """t(pick) / t(exhaustive best) - 1 per matrix."""
+    return np.asarray(times_pick, np.float64) / np.asarray(times_best, np.float64) - 1.0
+
+
+def evaluate(records, probs: np.ndarray) -> dict:
+    """Regrets of the network's pick against the exhaustive best (every
+    candidate of every format) and of the best single fixed format."""
+    best_t = np.array([r["exhaustive_ms"] for r in records])
+    tb = np.stack([format_times(r, "best") for r in records])
+    td = np.stack([format_times(r, "default") for r in records])
+    pick = np.argmax(probs, axis=1)
+    rows = np.arange(len(records))
+    r_default = regret(td[rows, pick], best_t)
+    r_tuned = regret(tb[rows, pick], best_t)
+    fixed = {}
+    for j, f in enumerate(FORMAT_ORDER):
+        rr = regret(td[:, j], best_t)
+        fixed[f] = float(np.mean(rr)) if np.all(np.isfinite(rr)) else math.inf
+    best_fixed = min(fixed, key=fixed.get)
+    return {
+        "count": len(records),
+        "mean_regret": float(np.mean(r_default)),
+        "median_regret": float(np.median(r_default)),
+        "p90_regret": float(np.quantile(r_default, 0.9)),
+        "within_5pct": float(np.mean(r_default <= 0.05)),
+        "mean_regret_tuned_format": float(np.mean(r_tuned)),
+        "format_accuracy": float(np.mean(pick == np.argmin(tb, axis=1))),
+        "fixed_format_mean_regret": fixed,
+        "best_fixed_format": best_fixed,
+        "best_fixed_mean_regret": fixed[best_fixed],
+        "picks": {f: int(np.sum(pick == j)) for j, f in enumerate(FORMAT_ORDER)},
+    }
+
+
+def train(X, P, epochs: int = 300, batch: int = 128, lr: float = 2e-3, seed: int = 0, mlp: MLP = None,
+          log=None) -> MLP:
+    """Adam on minibatches of (features, target distribution), on the GPU."""
+    X = np.ascontiguousarray(X, np.float32)
+    P = np.ascontiguousarray(P, np.float32)
+    net = mlp or MLP(seed)
+    rng = np.random.default_rng(seed)
+    n = X.shape[0]
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        tot = 0.0
+        for s in range(0, n, batch):
+            idx = order[s:s + batch]
+            tot += net.train_step(X[idx], P[idx], lr) * idx.size
+        if log and (ep % max(1, epochs // 10) == 0 or ep == epochs - 1):
+            log(f"epoch {ep} loss {tot / n:.4f}")
+    return net
+
+
+# ---- the sweep ------------------------------------------------------------------
+
+def sample_spec(index: int, seed: int = 0, max_nnz: int = 1 << 24, min_rows: int = 10_000,
+                max_rows: int = 4 << 20) -> "E.GenSpec":
+    """Matrix ``index`` of the selector dataset: families 2D/3D stencils,
+    uniform nnz/row and power law; rows log-uniform in [min_rows, max_rows],
+    mean nnz/row about 2-200, bandwidth fraction 0.001-1; rows reduced so that
+    nnz stays near or below max_nnz."""
+    rng = np.random.default_rng([seed, index])
+    fam = ["stencil2d", "stencil3d", "uniform", "powerlaw", "uniform", "powerlaw"][index % 6]
+    rows = int(math.exp(rng.uniform(math.log(min_rows), math.log(max_rows))))
+    vals = ["uniform", "normal"][index % 2]
+    if fam == "stencil2d":
+        rows = min(rows, max_nnz // 5)
+        return E.GenSpec("stencil2d", values=vals, n=max(8, int(math.sqrt(rows))), seed=index)
+    if fam == "stencil3d":
+        rows = min(rows, max_nnz // 27)
+        return E.GenSpec("stencil3d", values=vals, n=max(4, int(round(rows ** (1 / 3)))), seed=index)
+    bw = float(math.exp(rng.uniform(math.log(1e-3), 0.0)))
+    if fam == "uniform":
+        mean = math.exp(rng.uniform(math.log(2), math.log(200)))
+        spread = rng.uniform(0.0, 1.0)
+        rows = max(min_rows // 4, min(rows, int(max_nnz / mean)))
+        lo = max(1, int(mean * (1 - spread)))
+        hi = max(lo, int(mean * (1 + spread)))
+        return E.GenSpec("uniform", values=vals, rows=rows, nnz_lo=lo, nnz_hi=hi, bandwidth=bw, seed=index)
+    alpha = float(rng.uniform(1.5, 3.0))
+    lo = int(math.exp(rng.uniform(0.0, math.log(64))))
+    hi = int(min(rows, lo * 10 ** rng.uniform(1.0, 4.0)))
+    # E[min(X, hi)] for X with density ~ x^-alpha on [lo, inf): lo + int_lo^hi (lo/x)^(alpha-1) dx
+    if abs(alpha - 2.0) < 1e-9:
+        mean = lo * (1 + math.log(hi / lo))
+    else:
+        mean = lo + lo / (2.0 - alpha) * ((hi / lo) ** (2.0 - alpha) - 1.0)
+    rows = max(min_rows // 4, min(rows, int(max_nnz / max(mean, 1.0))))
+    hi = max(lo, min(hi, rows))
+    return E.GenSpec("powerlaw", values=vals, rows=rows, nnz_lo=lo, nnz_hi=hi, alpha=alpha, bandwidth=bw,
+                     seed=index)
+
+
+def measure(matrix: "E.DeviceMatrix", reps: int = 3, flush: bool = True) -> dict:
+    """Features and exhaustive candidate timings of one device matrix."""
+    vt = matrix.vector_dtype
+    x = E.DeviceArray.from_host(E.generate_vector(1, matrix.cols, vt))
+    y = E.DeviceArray(matrix.rows, vt)
+    t0 = time.perf_counter()
+    feats = features(matrix)
+    t_feat = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cands = E.autotune(matrix, x, y, reps=reps, flush=flush)
+    t_tune = time.perf_counter() - t0
+    best_ms, default_ms = {}, {}
+    for c in cands:
+        if c["status"] != 0 or not math.isfinite(c["ms"]):
+            continue
+        f = c["format"]
+        best_ms[f] = min(best_ms.get(f, math.inf), c["ms"])
+        if c["tuning"]["flags"] & L.MFP_TUNE_IS_DEFAULT:
+            default_ms[f] = c["ms"]
+    x.free()
+    y.free()
+    return {
+        "features": [float(v) for v in feats],
+        "best_ms": best_ms,
+        "default_ms": default_ms,
+        "exhaustive_ms": min(best_ms.values()),
+        "candidates": len(cands),
+        "feature_s": t_feat,
+        "autotune_s": t_tune,
+    }
+
+
+def _done_indices(path: str) -> set:
+    done = set()
+    if not os.path.exists(path):
+        return done
+    with open(path) as f:
+        for line in f:
+            try:
+                done.add(json.loads(line)["index"])
+            except (ValueError, KeyError):
+                break  # a torn last line: its matrix is measured again
+    return done
+
+
+def sweep(out: str, count: int, seed: int = 0, max_nnz: int = 1 << 24, reps: int = 3, start: int = 0,
+          deadline_s: float = None, log=print) -> int:
+    """Measure matrices start..count-1 into the JSONL file ``out``, skipping
+    those it already holds; returns how many were measured in this call."""
+    done = _done_indices(out)
+    t_start = time.perf_counter()
+    measured = 0
+    with open(out, "a") as f:
+        for i in range(start, count):
+            if i in done:
+                continue
+            if deadline_s is not None and time.perf_counter() - t_start > deadline_s:
+                log(f"deadline reached after {measured} matrices")
+                break
+            spec = sample_spec(i, seed, max_nnz)
+            rp, ci, va = E.generate(spec, np.float32)
+            cols = {"stencil2d": spec.n * spec.n, "stencil3d": spec.n ** 3}.get(spec.family, rp.size - 1)
+            A = E.DeviceMatrix.from_csr(rp, ci, va, cols)
+            rec = measure(A, reps=reps)
+            A.destroy()
+            rec.update({"index": i, "family": spec.family, "rows": int(rp.size - 1), "cols": int(cols),
+                        "nnz": int(ci.size), "spec": {k: getattr(spec, k) for k in
+                                                      ("n", "rows", "nnz_lo", "nnz_hi", "alpha", "bandwidth")}})
+            f.write(json.dumps(rec) + "\n")
+            f.flush()
+            measured += 1
+            if log and measured % 25 == 0:
+                log(f"{measured} measured (index {i}, {time.perf_counter() - t_start:.0f} s)")
+    return measured
+
+
+def load_records(path: str) -> list:
+    recs = {}
+    with open(path) as f:
+        for line in f:
+            try:
+                r = json.loads(line)
+            except ValueError:
+                break
+            recs[r["index"]] = r
+    return [recs[k] for k in sorted(recs)]
+
+
+def dataset(records):
+    X = np.array([r["features"] for r in records], np.float32)
+    T = np.stack([format_times(r, "best") for r in records])
+    return X, soft_targets(T)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m matrix_format_performance_amd.selector")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("sweep")
+    s.add_argument("--out", required=True)
+    s.add_argument("--count", type=int, default=2000)
+    s.add_argument("--start", type=int, default=0)
+    s.add_argument("--seed", type=int, default=0)
+    s.add_argument("--max-nnz", type=int, default=1 << 24)
+    s.add_argument("--reps", type=int, default=3)
+    s.add_argument("--deadline", type=float, default=None, help="stop after this many seconds")
+    t = sub.add_parser("train")
+    t.add_argument("--data", required=True)
+    t.add_argument("--out", required=True)
+    t.add_argument("--train", type=int, default=1600)
+    t.add_argument("--epochs", type=int, default=300)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args(argv)
+    E.init()
+    if a.cmd == "sweep":
+        n = sweep(a.out, a.count, a.seed, a.max_nnz, a.reps, a.start, a.deadline,
+                  log=lambda m: print(m, flush=True))
+        print(json.dumps({"measured": n, "total": len(_done_indices(a.out))}))
+        return 0
+    recs = load_records(a.data)
+    tr, te = recs[:a.train], recs[a.train:]
+    X, P = dataset(tr)
+    net = train(X, P, epochs=a.epochs, lr=a.lr, seed=a.seed, log=lambda m: print(m, flush=True))
+    np.save(a.out, net.params)
+    rep = {"train": evaluate(tr, net.predict(X))}
+    if te:
+        Xt, _ = dataset(te)
+        rep["held_out"] = evaluate(te, net.predict(Xt))
+        rep["held_out_bf16"] = evaluate(te, net.predict(Xt, bf16=True))
+    print(json.dumps(rep, indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -121,7 +121,8 @@ def test_autotune_sweeps_and_times_candidates(gpu):
     lens = np.sort(np.diff(rp.astype(np.int64)))
     q = lambda p: int(lens[int(p * (lens.size - 1))])
     widths = sorted({-(-ci.size // (rp.size - 1)), q(0.75), q(0.9), q(0.99)} - {0})
-    assert sorted(r["tuning"]["hyb_width"] for r in res if r["format"] == "hyb") == widths
+    # plus the default candidate, whose width 0 selects the hyb_percent rule
+    assert sorted(r["tuning"]["hyb_width"] for r in res if r["format"] == "hyb") == [0] + widths
     ok = [r for r in res if r["status"] == 0]
     assert ok and all(np.isfinite(r["ms"]) and r["ms"] > 0 for r in ok)
     assert res[0]["ms"] == min(r["ms"] for r in res)
