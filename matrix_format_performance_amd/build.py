@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmfp.so")
+APP = os.path.join(HERE, "mfp_bench")
 
 SOURCES = [
     "runtime.hip",
@@ -69,9 +70,26 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_app()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {APP}")
     return LIB
+
+
+def build_app() -> str:
+    """The C++ benchmark driver (app/mfp_bench.cpp) over the C ABI and the
+    reference-compatible header include/mfp_compat.hpp; host code only, so
+    g++ builds it (x86-64-v3 so it runs on any AVX2 host)."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = APP + ".tmp"
+    cmd = [cxx, "-O3", "-std=c++17", "-march=x86-64-v3", "-Wall", "-Wextra", "-Wno-unused-parameter",
+           os.path.join(HERE, "app", "mfp_bench.cpp"), "-o", tmp, "-L" + HERE, "-lmfp",
+           "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mfp_bench build failed:\n{r.stderr}")
+    os.replace(tmp, APP)
+    return APP
 
 
 if __name__ == "__main__":
