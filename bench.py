@@ -10,9 +10,10 @@ kernels run on.
 value = algorithmic bytes / mean step time, in GB/s, where
   B_alg = nnz*(s_v + 4) + (rows + 1)*4 + cols*s_x + rows*s_y
 (the CSR compulsory traffic; padding, COO row indices and repeated x reads
-count as overhead, not credit).  The format measured is the fastest of all
-formats on this workload (exhaustive selection, every format timed the same
-way), reported in config.format, with the per-format table alongside.
+count as overhead, not credit).  The plan measured is the fastest autotuner
+candidate (every format and its swept tunings, each timed the same way as
+the steps), reported in config.format / config.tuning, with the best
+candidate of every format in the "formats" table.
 
 cpu_baseline: the oracle's restatement of the reference CPU path
 (cpu_matrix_multiplier.cpp), timed on this host on the same matrix.
@@ -41,6 +42,17 @@ WORKLOADS = {
     "stencil3d_128": "3D 27-point stencil 128^3 (2,097,152 rows, 55,742,968 nnz), values N(0,1)",
     "stencil2d_256": "2D 5-point Laplacian 256^2 (65,536 rows, 326,656 nnz)",
 }
+
+
+def tuning_summary(fmt, t):
+    """The tuning fields that matter for a format."""
+    keys = {"csr_vector": ["lanes"], "csr_adaptive": ["nnz_per_wg", "long_row", "chunk"],
+            "ell": ["ell_width"], "hyb": ["hyb_width", "hyb_percent"], "hyb_atomic": ["hyb_width", "hyb_percent"],
+            "scoo": ["scoo_slice_size", "scoo_lanes", "scoo_slices", "scoo_pieces", "scoo_threads"]}.get(fmt, [])
+    out = {k: t[k] for k in keys}
+    if fmt == "scoo":
+        out["column_sorted"] = bool(t["flags"] & 4)
+    return out
 
 
 def b_alg(rows, cols, nnz, s_v, s_x):
@@ -101,6 +113,68 @@ def cpu_baseline(rp, ci, va, x, nbytes, threads):
     }
 
 
+def precision_study(mfp, timer, flush, reps, cache):
+    """FP32 plain form against BF16 / FP16 value storage (FP32 accumulation,
+    compensated row sums) on the 3D 27-point 128^3 stencil and the power-law
+    matrix: best time per value type, speed-up over FP32, and the error
+    against the GPU FP64 oracle of the stored values (arithmetic) and of the
+    FP32 values (arithmetic + quantisation)."""
+    out = {}
+    for wl in ("stencil3d_128", "powerlaw_1m"):
+        rp, ci, va = cache[wl] if wl in cache else mfp.workload(wl, np.float32)
+        rows, nnz = rp.size - 1, ci.size
+        A = mfp.DeviceMatrix.from_csr(rp, ci, va, rows)
+        x = mfp.generate_vector(1, rows, np.float32)
+        xd = mfp.DeviceArray.from_host(x)
+        yd = mfp.DeviceArray(rows, np.float32)
+        y64, ad = A.oracle_f64(xd)
+        res = {}
+        for vt in ("f32", "bf16", "f16"):
+            M = A if vt == "f32" else A.convert(vt)
+            s_v = 4 if vt == "f32" else 2
+            nb = b_alg(rows, rows, nnz, s_v, 4)
+            best = None
+            for fmt in (("csr_vector", "csr_adaptive", "ell") if vt == "f32" else ("csr_vector", "ell")):
+                try:
+                    p = mfp.Plan(M, fmt)
+                except mfp.MfpBudgetError:
+                    continue
+                time_plan(mfp, p, xd, yd, 1, flush, timer)
+                ms = statistics.median(time_plan(mfp, p, xd, yd, reps, flush, timer))
+                if best is None or ms < best[1]:
+                    if best is not None:
+                        best[2].destroy()
+                    best = (fmt, ms, p)
+                else:
+                    p.destroy()
+            fmt, ms, p = best
+            p.spmv(xd, yd)
+            mfp.synchronize()
+            total = mfp.residual(yd, y64, ad)
+            if vt == "f32":
+                arith = total
+            else:
+                ys, ads = M.oracle_f64(xd)
+                arith = mfp.residual(yd, ys, ads)
+                ys.free()
+                ads.free()
+            p.destroy()
+            res[vt] = {"format": fmt, "ms": round(ms, 5), "GB_per_s": round(nb / (ms * 1e-3) / 1e9, 1),
+                       "algorithmic_bytes": nb, "rel_l2_vs_fp64_stored": arith["rel_l2"],
+                       "max_row_err_vs_fp64_stored": arith["max_row_err"],
+                       "rel_l2_vs_fp64_fp32_values": total["rel_l2"]}
+            if vt != "f32":
+                res[vt]["speedup_vs_f32"] = round(res["f32"]["ms"] / ms, 3)
+                M.destroy()
+            log(f"[bench] precision {wl} {vt}: {fmt} {ms * 1e3:.1f} us "
+                f"err(stored) {arith['rel_l2']:.2e} err(total) {total['rel_l2']:.2e}")
+        for a in (xd, yd, y64, ad):
+            a.free()
+        A.destroy()
+        out[wl] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +188,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
+    ap.add_argument("--no-precision", action="store_true", help="skip the BF16 / FP16 study")
     ap.add_argument("--bandwidth", type=float, default=1.0,
                     help="power-law only: column window as a fraction of cols (study knob)")
     args = ap.parse_args()
@@ -152,31 +227,32 @@ def main():
     copy_gbs = measured_copy_gbs(mfp, timer)
     log(f"[bench] measured 16-byte copy: {copy_gbs:.1f} GB/s")
 
+    # exhaustive selection: every autotuner candidate of every format, timed
+    # the same way as the measured steps (flush, event pair, median)
     fmts = list(mfp.FORMATS) if args.format == "auto" else [args.format]
+    t_sel = time.time()
+    cands = mfp.autotune(A, xd, yd, formats=fmts, reps=args.select_reps, flush=flush)
+    t_sel = time.time() - t_sel
     table = {}
-    plans = {}
-    for f in fmts:
-        try:
-            kw = {}
-            if f in ("hyb", "hyb_atomic"):
-                kw = {"hyb_width": max(1, -(-nnz // rows))}  # tuned split: avg row length
-            p = mfp.Plan(A, f, **kw)
-        except mfp.MfpBudgetError as e:
-            table[f] = {"skipped": "layout above the memory budget"}
-            log(f"[bench] {f}: skipped ({e})")
+    for c in cands:  # fastest first
+        f = c["format"]
+        if c["status"] != 0 or not np.isfinite(c["ms"]):
+            table.setdefault(f, {"skipped": f"status {c['status']} (layout above the memory budget)"
+                                 if c["status"] == 3 else f"status {c['status']}"})
             continue
-        time_plan(mfp, p, xd, yd, 1, flush, timer)  # warm the code path
-        ms = time_plan(mfp, p, xd, yd, args.select_reps, flush, timer)
-        med = statistics.median(ms)
+        if "ms" in table.get(f, {}):
+            continue
+        med = c["ms"]
         table[f] = {"ms": round(med, 5), "GB_per_s": round(nbytes / (med * 1e-3) / 1e9, 1),
-                    "frac_hbm": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        plans[f] = p
-        log(f"[bench] {f:13s} {med * 1e3:9.2f} us  {table[f]['GB_per_s']:8.1f} GB/s")
-    best = min(plans, key=lambda f: table[f]["ms"])
-    plan = plans[best]
-    for f, p in plans.items():
-        if f != best:
-            p.destroy()
+                    "frac_hbm": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "tuning": tuning_summary(f, c["tuning"])}
+        log(f"[bench] {f:13s} {med * 1e3:9.2f} us  {table[f]['GB_per_s']:8.1f} GB/s  {table[f]['tuning']}")
+    ok = [c for c in cands if c["status"] == 0 and np.isfinite(c["ms"])]
+    best_c = ok[0]
+    best = best_c["format"]
+    plan = mfp.Plan(A, best, **best_c["tuning"])
+    log(f"[bench] exhaustive selection over {len(cands)} candidates in {t_sel:.1f}s: {best} "
+        f"{tuning_summary(best, best_c['tuning'])}")
 
     # correctness of the measured format on this very workload
     time_plan(mfp, plan, xd, yd, 1, flush, timer)
@@ -221,6 +297,8 @@ def main():
             "description": WORKLOADS[args.workload],
             "rows": rows, "cols": cols, "nnz": nnz,
             "format": best,
+            "tuning": tuning_summary(best, best_c["tuning"]),
+            "selection": f"exhaustive autotune, {len(cands)} candidates, {t_sel:.1f} s",
             "algorithmic_bytes": nbytes,
             "cache": "warm (no flush)" if not flush else "Infinity Cache evicted before every step",
             "median_ms": round(statistics.median(ms), 5),
@@ -242,6 +320,28 @@ def main():
             "traffic": traffic,
         },
     }
+    # host-to-host rate: upload CSR and x, build the plan, one SpMV, download y
+    plan_kw = best_c["tuning"]
+    mfp.synchronize()
+    t_h = time.perf_counter()
+    A2 = mfp.DeviceMatrix.from_csr(rp, ci, va, cols)
+    x2 = mfp.DeviceArray.from_host(x)
+    y2 = mfp.DeviceArray(rows, dtype)
+    p2 = mfp.Plan(A2, best, **plan_kw)
+    p2.spmv(x2, y2)
+    y_host = y2.to_host()
+    t_h = time.perf_counter() - t_h
+    assert y_host.shape == (rows,)
+    out["config"]["host_to_host_GB_per_s"] = round(nbytes / t_h / 1e9, 2)
+    out["config"]["host_to_host_ms"] = round(t_h * 1e3, 3)
+    for o in (p2, A2, x2, y2):
+        o.destroy() if hasattr(o, "destroy") else o.free()
+    plan.destroy()
+
+    if args.dtype == "f32" and not args.no_precision:
+        cache = {args.workload: (rp, ci, va)} if args.bandwidth >= 1.0 else {}
+        A.destroy()
+        out["precision"] = precision_study(mfp, timer, flush, args.select_reps, cache)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(rp, ci, va, x, nbytes, args.cpu_threads)

@@ -163,10 +163,16 @@ typedef struct {
   uint32_t scoo_threads;  /* SCOO workgroup size */
   uint64_t budget_bytes;  /* refuse layouts above this (0 = 90% of free HBM) */
   uint32_t flags;         /* MFP_TUNE_* */
+  uint32_t scoo_pieces;   /* column-sorted SCOO: workgroups per slice (0 = about two per CU in all) */
 } mfp_tuning;
 enum {
   MFP_TUNE_NO_OFFSET16 = 1, /* 16-bit-value ELL: keep 32-bit columns */
-  MFP_TUNE_IS_DEFAULT = 2   /* set by the autotuner on each format's default candidate; no effect */
+  MFP_TUNE_IS_DEFAULT = 2,  /* set by the autotuner on each format's default candidate; no effect */
+  /* SCOO with each slice's entries sorted by column and split into
+   * scoo_pieces workgroups; scoo_slice_size rows (a power of two) accumulate
+   * in LDS.  No reference counterpart (the reference's SCOO keeps row order,
+   * scoo_matrix_class.cpp:102). */
+  MFP_TUNE_SCOO_COLSORT = 4
 };
 
 typedef struct mfp_plan_s *mfp_plan;
@@ -216,6 +222,8 @@ typedef struct {
   uint32_t lanes;
   uint32_t nnz_per_wg;
   uint32_t column_bytes;     /* bytes per stored column index (2 with 16-bit offsets) */
+  uint32_t scoo_colsort;     /* column-sorted SCOO: lane_size holds the pieces per slice */
+  uint32_t scoo_packed;      /* column-sorted SCOO: column offset and row packed in one u32 */
 } mfp_plan_info;
 int mfp_plan_get_info(mfp_plan p, mfp_plan_info *out);
 

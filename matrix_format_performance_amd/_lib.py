@@ -54,11 +54,12 @@ class tuning(C.Structure):
     _fields_ = [("lanes", u32), ("nnz_per_wg", u32), ("long_row", u32), ("chunk", u32),
                 ("ell_width", u32), ("hyb_percent", dbl), ("hyb_width", u32), ("scoo_slices", u32),
                 ("scoo_slice_size", u32), ("scoo_lanes", u32), ("scoo_threads", u32),
-                ("budget_bytes", u64), ("flags", u32)]
+                ("budget_bytes", u64), ("flags", u32), ("scoo_pieces", u32)]
 
 
 MFP_TUNE_NO_OFFSET16 = 1
 MFP_TUNE_IS_DEFAULT = 2
+MFP_TUNE_SCOO_COLSORT = 4
 
 
 class candidate(C.Structure):
@@ -74,7 +75,7 @@ class plan_info(C.Structure):
                 ("device_bytes", u64), ("stored_elements", u64), ("ell_width", u32),
                 ("coo_elements", u64), ("blocks", u32), ("chunks", u32), ("slices_count", u32),
                 ("slice_size", u32), ("lane_size", u32), ("lanes", u32), ("nnz_per_wg", u32),
-                ("column_bytes", u32)]
+                ("column_bytes", u32), ("scoo_colsort", u32), ("scoo_packed", u32)]
 
 
 class measurement(C.Structure):

@@ -537,6 +537,220 @@ int scoo_slices_device(uint32_t rows, uint64_t nnz, const uint32_t *rp, uint32_t
 }
 
 // ---------------------------------------------------------------------------
+// Column-sorted sliced COO: slices of R consecutive rows; inside a slice the
+// entries are ordered by column (stable, so equal columns keep row order).
+// Each slice's entry range is cut into `pieces` equal-count pieces.  When
+// every piece's column span fits in 32 - log2(R) bits, an entry stores one
+// u32 (column - piece base) << rbits | local row; otherwise a u32 column and
+// a u16 local row.
+// ---------------------------------------------------------------------------
+__global__ void k_entry_slice(uint32_t rows, uint64_t nnz, const uint32_t *__restrict__ rp,
+                              uint32_t R, uint32_t *__restrict__ slice, uint32_t *__restrict__ lrow)
+{
+  uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; e < nnz; e += stride) {
+    const uint32_t r = row_of(rp, rows, (uint32_t)e);
+    slice[e] = r / R;
+    lrow[e] = r % R;
+  }
+}
+
+__global__ void k_slice_ptr(uint32_t S, uint32_t rows, uint32_t R, const uint32_t *__restrict__ rp,
+                            uint32_t *__restrict__ sp)
+{
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s <= S) sp[s] = rp[min((uint64_t)s * R, (uint64_t)rows)];
+}
+
+// piece q of slice s covers entries [sp[s] + len q / P, sp[s] + len (q + 1) / P)
+__device__ __forceinline__ uint32_t piece_begin(const uint32_t *__restrict__ sp, uint32_t s, uint32_t q,
+                                                uint32_t P)
+{
+  const uint32_t b = sp[s], len = sp[s + 1] - b;
+  return b + (uint32_t)(((uint64_t)len * q) / P);
+}
+
+__global__ void k_piece_base(uint32_t S, uint32_t P, const uint32_t *__restrict__ sp,
+                             const uint32_t *__restrict__ col, uint32_t *__restrict__ base,
+                             uint32_t *__restrict__ max_span)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S * P) return;
+  const uint32_t s = i / P, q = i % P;
+  const uint32_t e0 = piece_begin(sp, s, q, P), e1 = piece_begin(sp, s, q + 1, P);
+  if (e1 > e0) {
+    base[i] = col[e0];
+    atomicMax(max_span, col[e1 - 1] - col[e0]);
+  } else {
+    base[i] = 0;
+  }
+}
+
+template <typename T>
+__global__ void k_cs_scatter(uint64_t nnz, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ col,
+                             const uint32_t *__restrict__ lrow, const T *__restrict__ a, uint32_t S, uint32_t P,
+                             const uint32_t *__restrict__ sp, const uint32_t *__restrict__ base,
+                             const uint32_t *__restrict__ slice_sorted, uint32_t rbits, int packed,
+                             T *__restrict__ ev, uint32_t *__restrict__ ec, uint16_t *__restrict__ er)
+{
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < nnz; i += stride) {
+    const uint32_t src = perm[i];
+    ev[i] = a[src];
+    const uint32_t c = col[i];
+    if (packed) {
+      // the piece holding sorted position i
+      const uint32_t s = slice_sorted[i];
+      const uint32_t b = sp[s], len = sp[s + 1] - b;
+      uint32_t q = len ? (uint32_t)((((uint64_t)(i - b)) * P) / len) : 0u;
+      // piece_begin rounds down; step to the piece whose range holds i
+      while (q + 1 < P && piece_begin(sp, s, q + 1, P) <= i) q++;
+      while (q > 0 && piece_begin(sp, s, q, P) > i) q--;
+      ec[i] = ((c - base[(uint64_t)s * P + q]) << rbits) | lrow[src];
+    } else {
+      ec[i] = c;
+      er[i] = (uint16_t)lrow[src];
+    }
+  }
+}
+
+template <typename T>
+static int cs_scatter_t(uint64_t nnz, const uint32_t *perm, const uint32_t *col, const uint32_t *lrow,
+                        const void *a, uint32_t S, uint32_t P, const uint32_t *sp, const uint32_t *base,
+                        const uint32_t *slice_sorted, uint32_t rbits, int packed, void *ev, uint32_t *ec,
+                        uint16_t *er, hipStream_t s)
+{
+  k_cs_scatter<T><<<grid_for(nnz), 256, 0, s>>>(nnz, perm, col, lrow, (const T *)a, S, P, sp, base, slice_sorted,
+                                                rbits, packed, (T *)ev, ec, er);
+  MFP_HIP(hipGetLastError());
+  return MFP_OK;
+}
+
+int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t *rp,
+                         const uint32_t *ci, const void *val, uint32_t R, uint32_t P, scoo_cs_layout *out,
+                         hipStream_t s)
+{
+  MFP_ARG(R >= 1 && R <= 65536 && (R & (R - 1)) == 0, "column-sorted SCOO slice size must be a power of two <= 65536");
+  MFP_ARG(P >= 1, "pieces must be positive");
+  MFP_ARG(nnz < (1ull << 32), "nnz must fit in 32 bits");
+  *out = scoo_cs_layout{};
+  const uint32_t S = rows ? (uint32_t)(((uint64_t)rows + R - 1) / R) : 0u;
+  out->R = R;
+  out->P = P;
+  out->S = S;
+  uint32_t rbits = 0;
+  while ((1u << rbits) < R) rbits++;
+  out->rbits = rbits;
+  const size_t vs = dtype_size(dtype);
+  MFP_CHECK(dmalloc((void **)&out->sp, ((uint64_t)S + 1) * 4));
+  MFP_CHECK(dmalloc((void **)&out->base, std::max<uint64_t>(1, (uint64_t)S * P) * 4));
+  MFP_CHECK(dmalloc(&out->ev, std::max<uint64_t>(nnz, 1) * vs));
+  MFP_CHECK(dmalloc((void **)&out->ec, std::max<uint64_t>(nnz, 1) * 4));
+  if (S) {
+    k_slice_ptr<<<ceil_div_u32((uint64_t)S + 1, 256), 256, 0, s>>>(S, rows, R, rp, out->sp);
+    MFP_HIP(hipGetLastError());
+  }
+  if (nnz == 0) {
+    if (S) MFP_CHECK(fill_u32(out->base, 0, (uint64_t)S * P, s));
+    out->packed = 1;
+    MFP_HIP(hipStreamSynchronize(s));
+    return MFP_OK;
+  }
+  // (slice, column, index) sorted by column, then stably by slice
+  uint32_t *buf[7] = {};
+  for (int i = 0; i < 7; i++) {
+    const int st = dmalloc((void **)&buf[i], nnz * 4);
+    if (st != MFP_OK) {
+      for (int j = 0; j < i; j++) dfree(buf[j]);
+      return st;
+    }
+  }
+  const uint32_t ntiles = (uint32_t)((nnz + RX_TILE - 1) / RX_TILE);
+  uint32_t *hist = nullptr, *span = nullptr;
+  int st = dmalloc((void **)&hist, (uint64_t)ntiles * 256 * 4);
+  if (st == MFP_OK) st = dmalloc((void **)&span, 4);
+  auto cleanup = [&]() {
+    for (int i = 0; i < 7; i++) dfree(buf[i]);
+    dfree(hist);
+    dfree(span);
+  };
+  if (st != MFP_OK) {
+    cleanup();
+    return st;
+  }
+  uint32_t *lrow = buf[6];
+  k_entry_slice<<<grid_for(nnz), 256, 0, s>>>(rows, nnz, rp, R, buf[0], lrow);
+  MFP_HIP(hipGetLastError());
+  MFP_HIP(hipMemcpyAsync(buf[1], ci, nnz * 4, hipMemcpyDeviceToDevice, s));
+  k_iota<<<grid_for(nnz), 256, 0, s>>>(buf[2], nnz);
+  MFP_HIP(hipGetLastError());
+  uint32_t *cur[3] = {buf[0], buf[1], buf[2]};
+  uint32_t *nxt[3] = {buf[3], buf[4], buf[5]};
+  std::vector<std::pair<int, uint32_t>> passes;
+  for (uint32_t sh = 0; sh < bit_length(cols ? cols - 1 : 0); sh += 8) passes.push_back({1, sh});
+  for (uint32_t sh = 0; sh < bit_length(S ? S - 1 : 0); sh += 8) passes.push_back({0, sh});
+  for (auto [which, sh] : passes) {
+    k_radix_hist<<<ntiles, RX_T, 0, s>>>(cur[which], nnz, sh, hist, ntiles);
+    if (hipGetLastError() != hipSuccess) {
+      cleanup();
+      set_error("radix hist launch failed");
+      return MFP_ERR_HIP;
+    }
+    st = exclusive_scan_u32(hist, hist, (uint64_t)ntiles * 256, s, nullptr);
+    if (st != MFP_OK) {
+      cleanup();
+      return st;
+    }
+    k_radix_scatter<<<ntiles, RX_T, 0, s>>>(cur[which], cur[0], cur[1], cur[2], nxt[0], nxt[1], nxt[2], nnz, sh,
+                                             hist, ntiles);
+    if (hipGetLastError() != hipSuccess) {
+      cleanup();
+      set_error("radix scatter launch failed");
+      return MFP_ERR_HIP;
+    }
+    std::swap(cur, nxt);
+  }
+  // cur[0] = slice, cur[1] = column, cur[2] = source entry, in the new order
+  st = fill_u32(span, 0, 1, s);
+  if (st == MFP_OK && S) {
+    k_piece_base<<<ceil_div_u32((uint64_t)S * P, 256), 256, 0, s>>>(S, P, out->sp, cur[1], out->base, span);
+    if (hipGetLastError() != hipSuccess) st = MFP_ERR_HIP;
+  }
+  uint32_t hspan = 0;
+  if (st == MFP_OK && hipMemcpyAsync(&hspan, span, 4, hipMemcpyDeviceToHost, s) != hipSuccess) st = MFP_ERR_HIP;
+  if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
+  if (st != MFP_OK) {
+    cleanup();
+    set_error("column-sorted SCOO: piece bases failed");
+    return st == MFP_ERR_HIP ? MFP_ERR_HIP : st;
+  }
+  out->packed = rbits == 0 || (uint64_t)hspan < (1ull << (32 - rbits));
+  if (!out->packed) st = dmalloc((void **)&out->er, nnz * 2);
+  if (st == MFP_OK)
+    st = (dtype == MFP_F32 ? cs_scatter_t<float> : cs_scatter_t<double>)(
+        nnz, cur[2], cur[1], lrow, val, S, P, out->sp, out->base, cur[0], rbits, out->packed, out->ev, out->ec,
+        out->er, s);
+  if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("column-sorted SCOO build failed");
+    st = MFP_ERR_HIP;
+  }
+  cleanup();
+  return st;
+}
+
+void scoo_cs_free(scoo_cs_layout *l)
+{
+  dfree(l->sp);
+  dfree(l->base);
+  dfree(l->ev);
+  dfree(l->ec);
+  dfree(l->er);
+  *l = scoo_cs_layout{};
+}
+
+// ---------------------------------------------------------------------------
 // Row blocks (fill_row_blocks, gpu/csr_adaptive_spmv.cu:180-240) in parallel.
 // The greedy partition is a chain 0 -> close(0) -> ...: close(s) is the row
 // where a block started at s closes (or NONE when the loop ends first), and it

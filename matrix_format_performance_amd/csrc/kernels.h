@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "mfp_internal.h"
+
 namespace mfp {
 
 struct adaptive_args {
@@ -40,6 +42,8 @@ int launch_coo_add(int dtype, uint64_t n, const uint32_t *cr, const uint32_t *cc
 int launch_hyb_atomic(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const void *ev,
                       uint64_t ncoo, const uint32_t *cr, const uint32_t *cc, const void *cv,
                       const void *x, void *y, hipStream_t s);
+int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
+                   void *partial, hipStream_t s);
 int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t threads,
                 const uint32_t *cr, const uint32_t *cc, const void *cv, const uint32_t *sp,
                 const void *x, void *y, hipStream_t s);

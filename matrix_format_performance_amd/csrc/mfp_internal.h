@@ -90,6 +90,20 @@ int row_blocks_device(uint32_t rows, const uint32_t *row_ptr, uint32_t nnz_per_w
                       uint32_t **blocks_out, uint32_t *count_out, hipStream_t s);
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s,
                        uint32_t *total_out /* host, may be null */);
+// Column-sorted sliced COO layout (plans with MFP_TUNE_SCOO_COLSORT).
+struct scoo_cs_layout {
+  uint32_t R = 0, P = 0, S = 0, rbits = 0;
+  int packed = 0;
+  uint32_t *sp = nullptr;   // S + 1 entry offsets of the slices
+  uint32_t *base = nullptr; // S * P first column of each piece (packed form)
+  void *ev = nullptr;       // values in slice-major, column-sorted order
+  uint32_t *ec = nullptr;   // packed (column - base) << rbits | local row, or the column
+  uint16_t *er = nullptr;   // local row (unpacked form only)
+};
+int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t *rp,
+                         const uint32_t *ci, const void *val, uint32_t R, uint32_t P, scoo_cs_layout *out,
+                         hipStream_t s);
+void scoo_cs_free(scoo_cs_layout *l);
 int fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 int fill_bytes_zero(void *p, size_t bytes, hipStream_t s);
 int convert_values(int dtype_in, const void *in, int dtype_out, void *out, uint64_t n,

@@ -32,5 +32,9 @@ struct mfp_plan_s {
   bool off16 = false;
   void *lp_cols = nullptr;
   uint32_t *lp_base = nullptr;
+  // column-sorted SCOO (MFP_TUNE_SCOO_COLSORT)
+  bool colsort = false;
+  mfp::scoo_cs_layout cs;
+  void *cs_partial = nullptr;
 };
 

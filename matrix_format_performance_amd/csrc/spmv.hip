@@ -565,6 +565,83 @@ __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *_
 }
 
 // ---------------------------------------------------------------------------
+// Column-sorted SCOO: one workgroup per (slice, piece).  The slice's rows
+// accumulate in LDS (ds_add); entries come in column order, so the x reads
+// of a piece sweep one column range and consecutive lanes share x lines.
+// Logical block b = q * S + s keeps equal pieces q of all slices together on
+// one XCD, whose L2 then holds that piece's part of x.  With P = 1 the slice
+// is written to y; otherwise to partial[q][row] and k_scoo_cs_finish adds the
+// P partials of each row in q order.
+// ---------------------------------------------------------------------------
+template <typename T, bool PACKED>
+__global__ void k_scoo_cs(uint32_t rows, uint32_t S, uint32_t P, uint32_t R, uint32_t rbits,
+                          const uint32_t *__restrict__ sp, const uint32_t *__restrict__ pbase,
+                          const T *__restrict__ ev, const uint32_t *__restrict__ ec,
+                          const uint16_t *__restrict__ er, const T *__restrict__ x, T *__restrict__ out)
+{
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *acc = reinterpret_cast<T *>(smem_raw);
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t q = b / S, s = b % S;
+  const uint32_t r0 = s * R;
+  const uint32_t lim = min(R, rows - r0);
+  const uint32_t bt = blockDim.x, tid = threadIdx.x;
+  for (uint32_t i = tid; i < lim; i += bt) acc[i] = (T)0;
+  __syncthreads();
+  const uint32_t sb = sp[s], len = sp[s + 1] - sb;
+  const uint32_t e0 = sb + (uint32_t)(((uint64_t)len * q) / P);
+  const uint32_t e1 = sb + (uint32_t)(((uint64_t)len * (q + 1)) / P);
+  const uint32_t cbase = PACKED ? pbase[(uint64_t)s * P + q] : 0u;
+  const uint32_t rmask = (1u << rbits) - 1u;
+  uint32_t i = e0 + tid;
+  for (; i + 3 * bt < e1; i += 4 * bt) {
+    uint32_t c[4], lr[4];
+    T v[4], xv[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      const uint32_t w = ldg_stream(ec + i + k * bt);
+      v[k] = ldg_stream(ev + i + k * bt);
+      if constexpr (PACKED) {
+        c[k] = cbase + (w >> rbits);
+        lr[k] = w & rmask;
+      } else {
+        c[k] = w;
+        lr[k] = ldg_stream(er + i + k * bt);
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) xv[k] = x[c[k]];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) atomicAdd(acc + lr[k], v[k] * xv[k]);
+  }
+  for (; i < e1; i += bt) {
+    const uint32_t w = ldg_stream(ec + i);
+    uint32_t c, lr;
+    if constexpr (PACKED) {
+      c = cbase + (w >> rbits);
+      lr = w & rmask;
+    } else {
+      c = w;
+      lr = ldg_stream(er + i);
+    }
+    atomicAdd(acc + lr, ldg_stream(ev + i) * x[c]);
+  }
+  __syncthreads();
+  T *o = P == 1 ? out + r0 : out + (uint64_t)q * rows + r0;
+  for (uint32_t lr = tid; lr < lim; lr += bt) o[lr] = acc[lr];
+}
+
+template <typename T>
+__global__ void k_scoo_cs_finish(uint32_t rows, uint32_t P, const T *__restrict__ partial, T *__restrict__ y)
+{
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  T d = partial[r];
+  for (uint32_t q = 1; q < P; q++) d = d + partial[(uint64_t)q * rows + r];
+  y[r] = d;
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static int launched() { return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP); }
@@ -760,6 +837,46 @@ int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t thre
     k_scoo<double><<<real, threads, lds, s>>>(rows, ss, L, cr, cc, (const double *)cv, sp, (const double *)x, (double *)y);
   }
   return launched();
+}
+
+template <typename T, bool PACKED>
+static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
+                     void *partial, hipStream_t s)
+{
+  const size_t lds = (size_t)l.R * sizeof(T);
+  if (lds > 65536)
+    MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T, PACKED>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  T *out = l.P == 1 ? (T *)y : (T *)partial;
+  k_scoo_cs<T, PACKED><<<l.S * l.P, threads, lds, s>>>(rows, l.S, l.P, l.R, l.rbits, l.sp, l.base,
+                                                       (const T *)l.ev, l.ec, l.er, (const T *)x, out);
+  MFP_CHECK(launched());
+  if (l.P > 1) {
+    k_scoo_cs_finish<T><<<ceil_div_u32(rows, 256), 256, 0, s>>>(rows, l.P, (const T *)partial, (T *)y);
+    MFP_CHECK(launched());
+  }
+  return MFP_OK;
+}
+
+int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
+                   void *partial, hipStream_t s)
+{
+  if (rows == 0) return MFP_OK;
+  if (l.S * (uint64_t)l.P == 0) return MFP_OK;
+  const size_t lds = (size_t)l.R * dtype_size(dtype);
+  if (lds > 163840) {
+    set_error("column-sorted SCOO slice exceeds the LDS of a workgroup");
+    return MFP_ERR_ARG;
+  }
+  if (threads == 0 || threads > 1024 || threads % 64) {
+    set_error("SCOO workgroup size must be a multiple of 64 up to 1024");
+    return MFP_ERR_ARG;
+  }
+  if (dtype == MFP_F32)
+    return l.packed ? scoo_cs_t<float, true>(rows, l, threads, x, y, partial, s)
+                    : scoo_cs_t<float, false>(rows, l, threads, x, y, partial, s);
+  return l.packed ? scoo_cs_t<double, true>(rows, l, threads, x, y, partial, s)
+                  : scoo_cs_t<double, false>(rows, l, threads, x, y, partial, s);
 }
 
 } // namespace mfp

@@ -135,6 +135,52 @@ def test_scoo_geometries(gpu, ss, lanes, threads):
     _tol_check(y, rp, ci, va, x)
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("R,P", [(256, 1), (1024, 3), (4096, 0)])
+@pytest.mark.parametrize("name", CASES)
+def test_scoo_colsort_within_tolerance(gpu, name, dtype, R, P):
+    flags = gpu._lib.MFP_TUNE_SCOO_COLSORT
+    y, (rp, ci, va, x), info = _run(gpu, name, dtype, "scoo", flags=flags, scoo_slice_size=R, scoo_pieces=P)
+    rows = rp.size - 1
+    assert info["scoo_colsort"] == 1 and info["slice_size"] == R
+    assert info["slices_count"] == -(-rows // R)
+    if P:
+        assert info["lane_size"] == P
+    _tol_check(y, rp, ci, va, x)
+
+
+@pytest.mark.parametrize("dtype,R", [("float32", 32768), ("float64", 16384)])
+@pytest.mark.parametrize("P", [1, 2])
+def test_scoo_colsort_unpacked_when_columns_span_wide(gpu, dtype, R, P):
+    # a piece spanning >= 2^(32 - log2 R) columns keeps a separate u16 row
+    rp, ci, va = gpu.generate(gpu.GenSpec("uniform", values="uniform", rows=300000, nnz_lo=1, nnz_hi=5, seed=5),
+                              np.dtype(dtype))
+    cols = rp.size - 1
+    announce(f"scoo colsort unpacked dtype={dtype} R={R} P={P}")
+    x = gpu.generate_vector(3, cols, np.dtype(dtype))
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=R, scoo_pieces=P)
+    info = plan.info()
+    assert info["scoo_packed"] == 0
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(cols, np.dtype(dtype))
+    yd.fill(np.nan)
+    plan.spmv(xd, yd)
+    gpu.synchronize()
+    _tol_check(yd.to_host(), rp, ci, va, x)
+    # a narrower slice packs
+    p2 = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=1024, scoo_pieces=P)
+    assert p2.info()["scoo_packed"] == 1
+
+
+def test_scoo_colsort_rejects_bad_slices(gpu):
+    rp, ci, va, cols = case("stencil2d_64", "float32")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    for R in (1000, 65536):  # not a power of two; accumulator above 160 KB
+        with pytest.raises(gpu.MfpError):
+            gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=R)
+
+
 def test_ell_budget_refusal(gpu):
     # a width * rows layout above the budget is refused, not attempted
     rp, ci, va, cols = case("powerlaw_20k", "float32")

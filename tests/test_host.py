@@ -202,6 +202,8 @@ template <typename T> void all(const matrix_market::matrix_class &mtx)
   hybrid_matrix_class<T> hyb(csr);
   hyb.allocate(csr, 0.2);
   scoo_matrix_class<T> scoo(256, 65536, csr);
+  scoo_matrix_class scoo_deduced(256, 65536, csr);  // main.cpp:223 spells it without <T>
+  (void)scoo_deduced.slice_size;
   resizable_gpu_memory<T> A, A2, x, y;
   resizable_gpu_memory<unsigned int> c, r, c2;
   T *h = nullptr;

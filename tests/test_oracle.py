@@ -134,6 +134,10 @@ def test_hybrid_width_rule():
 def test_pow2_helpers():
     assert [oracle.next_pow_2(x) for x in (1, 2, 3, 5, 64, 65)] == [1, 2, 4, 8, 64, 128]
     assert [oracle.prev_pow_2(x) for x in (1, 2, 3, 5, 64, 65, 0)] == [1, 2, 2, 4, 64, 64, 0]
+    # exhaustively against bit arithmetic
+    for x in range(1, 1 << 14):
+        assert oracle.next_pow_2(x) == 1 << (x - 1).bit_length()
+        assert oracle.prev_pow_2(x) == 1 << (x.bit_length() - 1)
 
 
 @pytest.mark.parametrize("k", range(7))
