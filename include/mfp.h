@@ -71,8 +71,10 @@ int mfp_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int mfp_memcpy_d2d(void *dst, const void *src, size_t bytes, mfp_stream s);
 /* dst[i] = value (value given as double, stored as dtype) */
 int mfp_fill(int dtype, void *dst, uint64_t n, double value, mfp_stream s);
-/* Overwrite `bytes` of scratch so the 256 MB Infinity Cache holds no matrix
- * data before a timed run (pass >= 512 MB). */
+/* Read `bytes` of scratch so the 256 MB Infinity Cache holds no matrix data
+ * before a timed run (pass >= 512 MB).  The scratch is written once when it
+ * is allocated; the flush itself only reads, so it leaves no dirty lines for
+ * the timed kernel to write back. */
 int mfp_flush_cache(uint64_t bytes, mfp_stream s);
 /* dst = src with 16-byte nontemporal loads: the measured-copy reference for
  * HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters. */
@@ -163,14 +165,16 @@ typedef struct {
   uint32_t scoo_threads;  /* SCOO workgroup size */
   uint64_t budget_bytes;  /* refuse layouts above this (0 = 90% of free HBM) */
   uint32_t flags;         /* MFP_TUNE_* */
-  uint32_t scoo_pieces;   /* column-sorted SCOO: workgroups per slice (0 = about two per CU in all) */
+  uint32_t scoo_pieces;   /* column-sorted SCOO: work items per slice (0 = about two per CU in all) */
 } mfp_tuning;
 enum {
   MFP_TUNE_NO_OFFSET16 = 1, /* 16-bit-value ELL: keep 32-bit columns */
   MFP_TUNE_IS_DEFAULT = 2,  /* set by the autotuner on each format's default candidate; no effect */
-  /* SCOO with each slice's entries sorted by column and split into
-   * scoo_pieces workgroups; scoo_slice_size rows (a power of two) accumulate
-   * in LDS.  No reference counterpart (the reference's SCOO keeps row order,
+  /* SCOO with each slice's entries sorted by column and split into work
+   * items (about scoo_pieces per slice; 0 = about two per CU over the whole
+   * matrix); scoo_slice_size rows (a power of two) accumulate in LDS, and
+   * rows much longer than a slice's mean share get slices of their own.  No
+   * reference counterpart (the reference's SCOO keeps row order,
    * scoo_matrix_class.cpp:102). */
   MFP_TUNE_SCOO_COLSORT = 4
 };
@@ -222,8 +226,10 @@ typedef struct {
   uint32_t lanes;
   uint32_t nnz_per_wg;
   uint32_t column_bytes;     /* bytes per stored column index (2 with 16-bit offsets) */
-  uint32_t scoo_colsort;     /* column-sorted SCOO: lane_size holds the pieces per slice */
+  uint32_t scoo_colsort;     /* column-sorted SCOO: slices_count counts the long-row slices too,
+                                lane_size holds the work items */
   uint32_t scoo_packed;      /* column-sorted SCOO: column offset and row packed in one u32 */
+  uint32_t scoo_long_rows;   /* column-sorted SCOO: rows moved to long-row slices */
 } mfp_plan_info;
 int mfp_plan_get_info(mfp_plan p, mfp_plan_info *out);
 

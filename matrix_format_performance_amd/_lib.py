@@ -75,7 +75,7 @@ class plan_info(C.Structure):
                 ("device_bytes", u64), ("stored_elements", u64), ("ell_width", u32),
                 ("coo_elements", u64), ("blocks", u32), ("chunks", u32), ("slices_count", u32),
                 ("slice_size", u32), ("lane_size", u32), ("lanes", u32), ("nnz_per_wg", u32),
-                ("column_bytes", u32), ("scoo_colsort", u32), ("scoo_packed", u32)]
+                ("column_bytes", u32), ("scoo_colsort", u32), ("scoo_packed", u32), ("scoo_long_rows", u32)]
 
 
 class measurement(C.Structure):

@@ -537,48 +537,81 @@ int scoo_slices_device(uint32_t rows, uint64_t nnz, const uint32_t *rp, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// Column-sorted sliced COO: slices of R consecutive rows; inside a slice the
-// entries are ordered by column (stable, so equal columns keep row order).
-// Each slice's entry range is cut into `pieces` equal-count pieces.  When
-// every piece's column span fits in 32 - log2(R) bits, an entry stores one
-// u32 (column - piece base) << rbits | local row; otherwise a u32 column and
-// a u16 local row.
+// Column-sorted sliced COO.  Rows longer than a threshold T ("long rows")
+// are moved to their own slices, R of them per slice, in row order; every
+// other row stays in the slice of R consecutive rows that holds it.  Inside a
+// slice the entries are ordered by column (stable, so equal columns keep row
+// order).  Each slice is cut into work items of about equal entry counts.
+// When every item's column span fits in 32 - log2(R) bits, an entry stores
+// one u32 (column - item base) << log2(R) | local row; otherwise a u32 column
+// and a u16 local row.
 // ---------------------------------------------------------------------------
-__global__ void k_entry_slice(uint32_t rows, uint64_t nnz, const uint32_t *__restrict__ rp,
-                              uint32_t R, uint32_t *__restrict__ slice, uint32_t *__restrict__ lrow)
+__global__ void k_long_flags(uint32_t rows, const uint32_t *__restrict__ rp, uint32_t T,
+                             uint32_t *__restrict__ flag)
+{
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) flag[r] = rp[r + 1] - rp[r] > T ? 1u : 0u;
+}
+
+__global__ void k_long_list(uint32_t rows, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ rank,
+                            uint32_t *__restrict__ list)
+{
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows && flag[r]) list[rank[r]] = (uint32_t)r;
+}
+
+__global__ void k_long_mask(uint32_t rows, const uint32_t *__restrict__ flag, uint32_t *__restrict__ mask)
+{
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((uint64_t)w * 32 >= rows) return;
+  uint32_t m = 0;
+  for (uint32_t b = 0; b < 32 && (uint64_t)w * 32 + b < rows; b++) m |= flag[(uint64_t)w * 32 + b] << b;
+  mask[w] = m;
+}
+
+__global__ void k_entry_slice(uint32_t rows, uint64_t nnz, const uint32_t *__restrict__ rp, uint32_t R,
+                              uint32_t rbits, uint32_t S_reg, const uint32_t *__restrict__ flag,
+                              const uint32_t *__restrict__ rank, uint32_t *__restrict__ slice,
+                              uint32_t *__restrict__ lrow)
 {
   uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; e < nnz; e += stride) {
     const uint32_t r = row_of(rp, rows, (uint32_t)e);
-    slice[e] = r / R;
-    lrow[e] = r % R;
+    if (flag && flag[r]) {
+      const uint32_t j = rank[r];
+      slice[e] = S_reg + (j >> rbits);
+      lrow[e] = j & (R - 1u);
+    } else {
+      slice[e] = r >> rbits;
+      lrow[e] = r & (R - 1u);
+    }
   }
 }
 
-__global__ void k_slice_ptr(uint32_t S, uint32_t rows, uint32_t R, const uint32_t *__restrict__ rp,
-                            uint32_t *__restrict__ sp)
+// sp[s] = first sorted position whose slice is >= s
+__global__ void k_slice_bounds(uint32_t S, uint64_t nnz, const uint32_t *__restrict__ slice_sorted,
+                               uint32_t *__restrict__ sp)
 {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s <= S) sp[s] = rp[min((uint64_t)s * R, (uint64_t)rows)];
+  if (s > S) return;
+  uint64_t lo = 0, hi = nnz;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (slice_sorted[mid] < s)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  sp[s] = (uint32_t)lo;
 }
 
-// piece q of slice s covers entries [sp[s] + len q / P, sp[s] + len (q + 1) / P)
-__device__ __forceinline__ uint32_t piece_begin(const uint32_t *__restrict__ sp, uint32_t s, uint32_t q,
-                                                uint32_t P)
-{
-  const uint32_t b = sp[s], len = sp[s + 1] - b;
-  return b + (uint32_t)(((uint64_t)len * q) / P);
-}
-
-__global__ void k_piece_base(uint32_t S, uint32_t P, const uint32_t *__restrict__ sp,
-                             const uint32_t *__restrict__ col, uint32_t *__restrict__ base,
-                             uint32_t *__restrict__ max_span)
+__global__ void k_item_base(uint32_t NI, const uint4 *__restrict__ items, const uint32_t *__restrict__ col,
+                            uint32_t *__restrict__ base, uint32_t *__restrict__ max_span)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= S * P) return;
-  const uint32_t s = i / P, q = i % P;
-  const uint32_t e0 = piece_begin(sp, s, q, P), e1 = piece_begin(sp, s, q + 1, P);
+  if (i >= NI) return;
+  const uint32_t e0 = items[i].y, e1 = items[i].z;
   if (e1 > e0) {
     base[i] = col[e0];
     atomicMax(max_span, col[e1 - 1] - col[e0]);
@@ -589,10 +622,9 @@ __global__ void k_piece_base(uint32_t S, uint32_t P, const uint32_t *__restrict_
 
 template <typename T>
 __global__ void k_cs_scatter(uint64_t nnz, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ col,
-                             const uint32_t *__restrict__ lrow, const T *__restrict__ a, uint32_t S, uint32_t P,
-                             const uint32_t *__restrict__ sp, const uint32_t *__restrict__ base,
-                             const uint32_t *__restrict__ slice_sorted, uint32_t rbits, int packed,
-                             T *__restrict__ ev, uint32_t *__restrict__ ec, uint16_t *__restrict__ er)
+                             const uint32_t *__restrict__ lrow, const T *__restrict__ a, uint32_t NI,
+                             const uint4 *__restrict__ items, const uint32_t *__restrict__ base, uint32_t rbits,
+                             int packed, T *__restrict__ ev, uint32_t *__restrict__ ec, uint16_t *__restrict__ er)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -601,14 +633,17 @@ __global__ void k_cs_scatter(uint64_t nnz, const uint32_t *__restrict__ perm, co
     ev[i] = a[src];
     const uint32_t c = col[i];
     if (packed) {
-      // the piece holding sorted position i
-      const uint32_t s = slice_sorted[i];
-      const uint32_t b = sp[s], len = sp[s + 1] - b;
-      uint32_t q = len ? (uint32_t)((((uint64_t)(i - b)) * P) / len) : 0u;
-      // piece_begin rounds down; step to the piece whose range holds i
-      while (q + 1 < P && piece_begin(sp, s, q + 1, P) <= i) q++;
-      while (q > 0 && piece_begin(sp, s, q, P) > i) q--;
-      ec[i] = ((c - base[(uint64_t)s * P + q]) << rbits) | lrow[src];
+      // the item holding sorted position i: last item with e0 <= i (items
+      // are in position order and cover every position)
+      uint32_t lo = 0, hi = NI;
+      while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (items[mid].y <= i)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      ec[i] = ((c - base[lo]) << rbits) | lrow[src];
     } else {
       ec[i] = c;
       er[i] = (uint16_t)lrow[src];
@@ -618,132 +653,181 @@ __global__ void k_cs_scatter(uint64_t nnz, const uint32_t *__restrict__ perm, co
 
 template <typename T>
 static int cs_scatter_t(uint64_t nnz, const uint32_t *perm, const uint32_t *col, const uint32_t *lrow,
-                        const void *a, uint32_t S, uint32_t P, const uint32_t *sp, const uint32_t *base,
-                        const uint32_t *slice_sorted, uint32_t rbits, int packed, void *ev, uint32_t *ec,
-                        uint16_t *er, hipStream_t s)
+                        const void *a, uint32_t NI, const uint4 *items, const uint32_t *base, uint32_t rbits,
+                        int packed, void *ev, uint32_t *ec, uint16_t *er, hipStream_t s)
 {
-  k_cs_scatter<T><<<grid_for(nnz), 256, 0, s>>>(nnz, perm, col, lrow, (const T *)a, S, P, sp, base, slice_sorted,
-                                                rbits, packed, (T *)ev, ec, er);
+  k_cs_scatter<T><<<grid_for(nnz), 256, 0, s>>>(nnz, perm, col, lrow, (const T *)a, NI, items, base, rbits, packed,
+                                                (T *)ev, ec, er);
   MFP_HIP(hipGetLastError());
   return MFP_OK;
 }
 
 int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t *rp,
-                         const uint32_t *ci, const void *val, uint32_t R, uint32_t P, scoo_cs_layout *out,
-                         hipStream_t s)
+                         const uint32_t *ci, const void *val, uint32_t R, uint32_t long_threshold,
+                         uint64_t item_entries, scoo_cs_layout *out, hipStream_t s)
 {
-  MFP_ARG(R >= 1 && R <= 65536 && (R & (R - 1)) == 0, "column-sorted SCOO slice size must be a power of two <= 65536");
-  MFP_ARG(P >= 1, "pieces must be positive");
+  MFP_ARG(R >= 1 && R <= 65536 && (R & (R - 1)) == 0,
+          "column-sorted SCOO slice size must be a power of two <= 65536");
   MFP_ARG(nnz < (1ull << 32), "nnz must fit in 32 bits");
+  MFP_ARG(item_entries >= 1, "entries per work item must be positive");
   *out = scoo_cs_layout{};
-  const uint32_t S = rows ? (uint32_t)(((uint64_t)rows + R - 1) / R) : 0u;
-  out->R = R;
-  out->P = P;
-  out->S = S;
   uint32_t rbits = 0;
   while ((1u << rbits) < R) rbits++;
+  out->R = R;
   out->rbits = rbits;
+  const uint32_t S_reg = rows ? (uint32_t)(((uint64_t)rows + R - 1) / R) : 0u;
+  out->S_reg = S_reg;
   const size_t vs = dtype_size(dtype);
-  MFP_CHECK(dmalloc((void **)&out->sp, ((uint64_t)S + 1) * 4));
-  MFP_CHECK(dmalloc((void **)&out->base, std::max<uint64_t>(1, (uint64_t)S * P) * 4));
-  MFP_CHECK(dmalloc(&out->ev, std::max<uint64_t>(nnz, 1) * vs));
-  MFP_CHECK(dmalloc((void **)&out->ec, std::max<uint64_t>(nnz, 1) * 4));
-  if (S) {
-    k_slice_ptr<<<ceil_div_u32((uint64_t)S + 1, 256), 256, 0, s>>>(S, rows, R, rp, out->sp);
-    MFP_HIP(hipGetLastError());
+  std::vector<void *> tmp;
+  auto scratch = [&](void **p, uint64_t bytes) {
+    const int st = dmalloc(p, std::max<uint64_t>(bytes, 4));
+    if (st == MFP_OK) tmp.push_back(*p);
+    return st;
+  };
+  auto done = [&](int st) {
+    for (void *p : tmp) dfree(p);
+    if (st != MFP_OK) scoo_cs_free(out);
+    return st;
+  };
+  int st = MFP_OK;
+  // long rows: flag, rank (exclusive scan) and list
+  uint32_t *flag = nullptr, *rank = nullptr;
+  uint32_t NL = 0;
+  if (rows && long_threshold != 0xFFFFFFFFu) {
+    st = scratch((void **)&flag, (uint64_t)rows * 4);
+    if (st == MFP_OK) st = scratch((void **)&rank, (uint64_t)rows * 4);
+    if (st != MFP_OK) return done(st);
+    k_long_flags<<<ceil_div_u32(rows, 256), 256, 0, s>>>(rows, rp, long_threshold, flag);
+    if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
+    st = exclusive_scan_u32(flag, rank, rows, s, &NL);
+    if (st != MFP_OK) return done(st);
   }
-  if (nnz == 0) {
-    if (S) MFP_CHECK(fill_u32(out->base, 0, (uint64_t)S * P, s));
-    out->packed = 1;
-    MFP_HIP(hipStreamSynchronize(s));
-    return MFP_OK;
+  out->NL = NL;
+  out->S_long = (uint32_t)(((uint64_t)NL + R - 1) / R);
+  const uint32_t S = S_reg + out->S_long;
+  out->S = S;
+  st = dmalloc((void **)&out->long_rows, std::max<uint64_t>(NL, 1) * 4);
+  if (st == MFP_OK) st = dmalloc((void **)&out->sp, ((uint64_t)S + 1) * 4);
+  if (st == MFP_OK && NL) st = dmalloc((void **)&out->long_mask, ((uint64_t)rows + 31) / 32 * 4);
+  if (st != MFP_OK) return done(st);
+  if (NL) {
+    k_long_list<<<ceil_div_u32(rows, 256), 256, 0, s>>>(rows, flag, rank, out->long_rows);
+    k_long_mask<<<ceil_div_u32(((uint64_t)rows + 31) / 32, 256), 256, 0, s>>>(rows, flag, out->long_mask);
+    if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
   }
-  // (slice, column, index) sorted by column, then stably by slice
+  // (slice, column, source entry) sorted by column, then stably by slice
   uint32_t *buf[7] = {};
-  for (int i = 0; i < 7; i++) {
-    const int st = dmalloc((void **)&buf[i], nnz * 4);
-    if (st != MFP_OK) {
-      for (int j = 0; j < i; j++) dfree(buf[j]);
-      return st;
-    }
-  }
+  for (int i = 0; i < 7 && st == MFP_OK; i++) st = scratch((void **)&buf[i], nnz * 4);
   const uint32_t ntiles = (uint32_t)((nnz + RX_TILE - 1) / RX_TILE);
   uint32_t *hist = nullptr, *span = nullptr;
-  int st = dmalloc((void **)&hist, (uint64_t)ntiles * 256 * 4);
-  if (st == MFP_OK) st = dmalloc((void **)&span, 4);
-  auto cleanup = [&]() {
-    for (int i = 0; i < 7; i++) dfree(buf[i]);
-    dfree(hist);
-    dfree(span);
-  };
-  if (st != MFP_OK) {
-    cleanup();
-    return st;
-  }
+  if (st == MFP_OK) st = scratch((void **)&hist, (uint64_t)std::max<uint32_t>(ntiles, 1) * 256 * 4);
+  if (st == MFP_OK) st = scratch((void **)&span, 4);
+  if (st != MFP_OK) return done(st);
   uint32_t *lrow = buf[6];
-  k_entry_slice<<<grid_for(nnz), 256, 0, s>>>(rows, nnz, rp, R, buf[0], lrow);
-  MFP_HIP(hipGetLastError());
-  MFP_HIP(hipMemcpyAsync(buf[1], ci, nnz * 4, hipMemcpyDeviceToDevice, s));
-  k_iota<<<grid_for(nnz), 256, 0, s>>>(buf[2], nnz);
-  MFP_HIP(hipGetLastError());
   uint32_t *cur[3] = {buf[0], buf[1], buf[2]};
   uint32_t *nxt[3] = {buf[3], buf[4], buf[5]};
-  std::vector<std::pair<int, uint32_t>> passes;
-  for (uint32_t sh = 0; sh < bit_length(cols ? cols - 1 : 0); sh += 8) passes.push_back({1, sh});
-  for (uint32_t sh = 0; sh < bit_length(S ? S - 1 : 0); sh += 8) passes.push_back({0, sh});
-  for (auto [which, sh] : passes) {
-    k_radix_hist<<<ntiles, RX_T, 0, s>>>(cur[which], nnz, sh, hist, ntiles);
-    if (hipGetLastError() != hipSuccess) {
-      cleanup();
-      set_error("radix hist launch failed");
-      return MFP_ERR_HIP;
+  if (nnz) {
+    k_entry_slice<<<grid_for(nnz), 256, 0, s>>>(rows, nnz, rp, R, rbits, S_reg, NL ? flag : nullptr, rank, buf[0],
+                                                 lrow);
+    if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
+    if (hipMemcpyAsync(buf[1], ci, nnz * 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return done(MFP_ERR_HIP);
+    k_iota<<<grid_for(nnz), 256, 0, s>>>(buf[2], nnz);
+    if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
+    std::vector<std::pair<int, uint32_t>> passes;
+    for (uint32_t sh = 0; sh < bit_length(cols ? cols - 1 : 0); sh += 8) passes.push_back({1, sh});
+    for (uint32_t sh = 0; sh < bit_length(S ? S - 1 : 0); sh += 8) passes.push_back({0, sh});
+    for (auto [which, sh] : passes) {
+      k_radix_hist<<<ntiles, RX_T, 0, s>>>(cur[which], nnz, sh, hist, ntiles);
+      if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
+      st = exclusive_scan_u32(hist, hist, (uint64_t)ntiles * 256, s, nullptr);
+      if (st != MFP_OK) return done(st);
+      k_radix_scatter<<<ntiles, RX_T, 0, s>>>(cur[which], cur[0], cur[1], cur[2], nxt[0], nxt[1], nxt[2], nnz, sh,
+                                               hist, ntiles);
+      if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
+      std::swap(cur, nxt);
     }
-    st = exclusive_scan_u32(hist, hist, (uint64_t)ntiles * 256, s, nullptr);
-    if (st != MFP_OK) {
-      cleanup();
-      return st;
-    }
-    k_radix_scatter<<<ntiles, RX_T, 0, s>>>(cur[which], cur[0], cur[1], cur[2], nxt[0], nxt[1], nxt[2], nnz, sh,
-                                             hist, ntiles);
-    if (hipGetLastError() != hipSuccess) {
-      cleanup();
-      set_error("radix scatter launch failed");
-      return MFP_ERR_HIP;
-    }
-    std::swap(cur, nxt);
   }
-  // cur[0] = slice, cur[1] = column, cur[2] = source entry, in the new order
+  // slice boundaries, then work items on the host
+  k_slice_bounds<<<ceil_div_u32((uint64_t)S + 1, 256), 256, 0, s>>>(S, nnz, cur[0], out->sp);
+  if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
+  std::vector<uint32_t> hsp((size_t)S + 1);
+  if (hipMemcpyAsync(hsp.data(), out->sp, hsp.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(MFP_ERR_HIP);
+  std::vector<uint4> items;
+  std::vector<uint4> fin; // finish blocks {slice, first partial slot, slots, first row in the slice}
+  uint32_t slots = 0;
+  for (uint32_t sl = 0; sl < S; sl++) {
+    const uint64_t E = hsp[sl + 1] - hsp[sl];
+    const uint32_t n = (uint32_t)std::max<uint64_t>(1, (E + item_entries / 2) / item_entries);
+    // long-row slices add into y with atomics; an ordinary slice with one
+    // item writes y, with several it fills partial slots
+    const uint32_t mode = sl >= S_reg ? CS_LONG : (n == 1 ? CS_DIRECT : slots);
+    if (sl < S_reg && n > 1) {
+      const uint32_t lim = std::min<uint64_t>(R, (uint64_t)rows - (uint64_t)sl * R);
+      for (uint32_t r0 = 0; r0 < lim; r0 += CS_FIN_ROWS) fin.push_back(make_uint4(sl, slots, n, r0));
+    }
+    for (uint32_t k = 0; k < n; k++) {
+      const uint32_t e0 = hsp[sl] + (uint32_t)(E * k / n), e1 = hsp[sl] + (uint32_t)(E * (k + 1) / n);
+      items.push_back(make_uint4(sl, e0, e1, mode < CS_DIRECT ? mode + k : mode));
+    }
+    if (mode < CS_DIRECT) slots += n;
+  }
+  out->NI = (uint32_t)items.size();
+  out->NF = (uint32_t)fin.size();
+  out->slots = slots;
+  st = dmalloc((void **)&out->items, std::max<size_t>(items.size(), 1) * sizeof(uint4));
+  if (st == MFP_OK) st = dmalloc((void **)&out->fin, std::max<size_t>(fin.size(), 1) * sizeof(uint4));
+  if (st == MFP_OK) st = dmalloc((void **)&out->base, std::max<size_t>(items.size(), 1) * 4);
+  if (st == MFP_OK && slots) st = dmalloc(&out->partial, (uint64_t)slots * R * vs);
+  if (st == MFP_OK) st = dmalloc(&out->ev, std::max<uint64_t>(nnz, 1) * vs);
+  if (st == MFP_OK) st = dmalloc((void **)&out->ec, std::max<uint64_t>(nnz, 1) * 4);
+  if (st != MFP_OK) return done(st);
+  if (!items.empty() &&
+      hipMemcpyAsync(out->items, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice, s) != hipSuccess)
+    return done(MFP_ERR_HIP);
+  if (!fin.empty() &&
+      hipMemcpyAsync(out->fin, fin.data(), fin.size() * sizeof(uint4), hipMemcpyHostToDevice, s) != hipSuccess)
+    return done(MFP_ERR_HIP);
   st = fill_u32(span, 0, 1, s);
-  if (st == MFP_OK && S) {
-    k_piece_base<<<ceil_div_u32((uint64_t)S * P, 256), 256, 0, s>>>(S, P, out->sp, cur[1], out->base, span);
-    if (hipGetLastError() != hipSuccess) st = MFP_ERR_HIP;
+  if (st != MFP_OK) return done(st);
+  if (out->NI) {
+    k_item_base<<<ceil_div_u32(out->NI, 256), 256, 0, s>>>(out->NI, out->items, cur[1], out->base, span);
+    if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
   }
   uint32_t hspan = 0;
-  if (st == MFP_OK && hipMemcpyAsync(&hspan, span, 4, hipMemcpyDeviceToHost, s) != hipSuccess) st = MFP_ERR_HIP;
-  if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
-  if (st != MFP_OK) {
-    cleanup();
-    set_error("column-sorted SCOO: piece bases failed");
-    return st == MFP_ERR_HIP ? MFP_ERR_HIP : st;
-  }
+  if (hipMemcpyAsync(&hspan, span, 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return done(MFP_ERR_HIP);
   out->packed = rbits == 0 || (uint64_t)hspan < (1ull << (32 - rbits));
-  if (!out->packed) st = dmalloc((void **)&out->er, nnz * 2);
-  if (st == MFP_OK)
-    st = (dtype == MFP_F32 ? cs_scatter_t<float> : cs_scatter_t<double>)(
-        nnz, cur[2], cur[1], lrow, val, S, P, out->sp, out->base, cur[0], rbits, out->packed, out->ev, out->ec,
-        out->er, s);
-  if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) {
-    set_error("column-sorted SCOO build failed");
-    st = MFP_ERR_HIP;
+  if (!out->packed) {
+    st = dmalloc((void **)&out->er, std::max<uint64_t>(nnz, 1) * 2);
+    if (st != MFP_OK) return done(st);
   }
-  cleanup();
-  return st;
+  if (nnz) {
+    st = (dtype == MFP_F32 ? cs_scatter_t<float> : cs_scatter_t<double>)(
+        nnz, cur[2], cur[1], lrow, val, out->NI, out->items, out->base, rbits, out->packed, out->ev, out->ec, out->er,
+        s);
+    if (st != MFP_OK) return done(st);
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("column-sorted SCOO build failed");
+    return done(MFP_ERR_HIP);
+  }
+  out->device_bytes = nnz * (vs + 4 + (out->packed ? 0 : 2)) + ((uint64_t)S + 1) * 4 +
+                      (uint64_t)out->NI * 20 + (uint64_t)out->NF * 16 + (uint64_t)NL * 4 +
+                      (NL ? ((uint64_t)rows + 31) / 32 * 4 : 0) + (uint64_t)slots * R * vs;
+  return done(MFP_OK);
 }
 
 void scoo_cs_free(scoo_cs_layout *l)
 {
   dfree(l->sp);
+  dfree(l->long_rows);
+  dfree(l->long_mask);
+  dfree(l->items);
+  dfree(l->fin);
   dfree(l->base);
+  dfree(l->partial);
   dfree(l->ev);
   dfree(l->ec);
   dfree(l->er);

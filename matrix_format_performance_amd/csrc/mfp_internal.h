@@ -91,18 +91,32 @@ int row_blocks_device(uint32_t rows, const uint32_t *row_ptr, uint32_t nnz_per_w
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t s,
                        uint32_t *total_out /* host, may be null */);
 // Column-sorted sliced COO layout (plans with MFP_TUNE_SCOO_COLSORT).
+// Slices 0..S_reg-1 hold R consecutive rows each; slices S_reg..S-1 hold the
+// long rows, R per slice in row order (long_rows[] gives their rows,
+// long_mask[] marks them).  A work item {slice, first entry, end entry, mode}
+// is one workgroup; mode is a partial slot, CS_DIRECT (the item writes the
+// slice's rows of y) or CS_LONG (it adds into y with atomics).  fin[] lists
+// finish blocks {slice, first slot, slots, first row} of CS_FIN_ROWS rows.
+constexpr uint32_t CS_DIRECT = 0xFFFFFFFEu, CS_LONG = 0xFFFFFFFFu, CS_FIN_ROWS = 256;
 struct scoo_cs_layout {
-  uint32_t R = 0, P = 0, S = 0, rbits = 0;
+  uint32_t R = 0, rbits = 0, S_reg = 0, S_long = 0, S = 0, NL = 0;
+  uint32_t NI = 0, NF = 0, slots = 0;
   int packed = 0;
-  uint32_t *sp = nullptr;   // S + 1 entry offsets of the slices
-  uint32_t *base = nullptr; // S * P first column of each piece (packed form)
-  void *ev = nullptr;       // values in slice-major, column-sorted order
-  uint32_t *ec = nullptr;   // packed (column - base) << rbits | local row, or the column
-  uint16_t *er = nullptr;   // local row (unpacked form only)
+  uint32_t *sp = nullptr;        // S + 1 entry offsets of the slices
+  uint32_t *long_rows = nullptr; // NL
+  uint32_t *long_mask = nullptr; // ceil(rows / 32) bit words, when NL > 0
+  uint4 *items = nullptr;        // NI
+  uint4 *fin = nullptr;          // NF
+  uint32_t *base = nullptr;      // NI first column of each item (packed form)
+  void *partial = nullptr;       // slots * R
+  void *ev = nullptr;            // values in slice-major, column-sorted order
+  uint32_t *ec = nullptr;        // packed (column - base) << rbits | local row, or the column
+  uint16_t *er = nullptr;        // local row (unpacked form only)
+  uint64_t device_bytes = 0;
 };
 int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t *rp,
-                         const uint32_t *ci, const void *val, uint32_t R, uint32_t P, scoo_cs_layout *out,
-                         hipStream_t s);
+                         const uint32_t *ci, const void *val, uint32_t R, uint32_t long_threshold,
+                         uint64_t item_entries, scoo_cs_layout *out, hipStream_t s);
 void scoo_cs_free(scoo_cs_layout *l);
 int fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 int fill_bytes_zero(void *p, size_t bytes, hipStream_t s);

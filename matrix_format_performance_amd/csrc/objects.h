@@ -35,6 +35,5 @@ struct mfp_plan_s {
   // column-sorted SCOO (MFP_TUNE_SCOO_COLSORT)
   bool colsort = false;
   mfp::scoo_cs_layout cs;
-  void *cs_partial = nullptr;
 };
 

@@ -144,7 +144,6 @@ static int plan_free_arrays(mfp_plan p)
   dfree(p->lp_cols);
   dfree(p->lp_base);
   scoo_cs_free(&p->cs);
-  dfree(p->cs_partial);
   free_adaptive_layout(&p->ad);
   return MFP_OK;
 }
@@ -369,9 +368,12 @@ int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *t)
   return MFP_OK;
 }
 
-// Column-sorted SCOO: slice_size rows per slice (a power of two; the slice
-// accumulates in LDS), pieces per slice from the tuning or, when 0, enough
-// to give about two workgroups per CU.
+// Column-sorted SCOO: slice_size rows per slice (a power of two; a slice
+// accumulates in LDS).  Rows longer than max(256, mean entries per slice /
+// 128) get their own slices: in an ordinary slice such a row would put many
+// lanes of every wave on one LDS address.  Work items take about
+// nnz / (2 CUs) entries each (at least 2048), or nnz / (slices * pieces)
+// when scoo_pieces is set.
 static int plan_create_scoo_cs(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipStream_t s)
 {
   const size_t vs = dtype_size(m->dtype);
@@ -380,29 +382,22 @@ static int plan_create_scoo_cs(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t,
     set_error("column-sorted SCOO slice_size must be a power of two whose accumulator fits in 160 KB of LDS");
     return MFP_ERR_ARG;
   }
-  const uint32_t S = m->rows ? (uint32_t)(((uint64_t)m->rows + R - 1) / R) : 0u;
-  uint32_t P = t.scoo_pieces;
-  if (P == 0) {
-    const uint32_t target = 2u * (uint32_t)std::max(1, props().cu_count);
-    P = S ? std::max<uint32_t>(1, (target + S - 1) / S) : 1u;
-    // keep at least 1024 entries per piece
-    const uint64_t per = S ? m->nnz / S : 0;
-    while (P > 1 && per / P < 1024) P--;
-  }
-  if ((uint64_t)S * P > 0x7FFFFFFFull) {
-    set_error("column-sorted SCOO: too many pieces");
-    return MFP_ERR_ARG;
-  }
+  const uint64_t S = m->rows ? ((uint64_t)m->rows + R - 1) / R : 0u;
+  const uint64_t per_slice = S ? m->nnz / S : 0;
+  const uint32_t T = (uint32_t)std::max<uint64_t>(256, per_slice / 128);
+  uint64_t item;
+  if (t.scoo_pieces)
+    item = std::max<uint64_t>(1, per_slice / t.scoo_pieces);
+  else
+    item = std::max<uint64_t>(2048, m->nnz / (2u * (uint64_t)std::max(1, props().cu_count)));
   p->colsort = true;
   p->scoo_threads = t.scoo_threads ? t.scoo_threads : 512;
-  MFP_CHECK(scoo_cs_build_device(m->dtype, m->rows, m->cols, m->nnz, m->row_ptr, m->columns, m->values, R, P,
+  MFP_CHECK(scoo_cs_build_device(m->dtype, m->rows, m->cols, m->nnz, m->row_ptr, m->columns, m->values, R, T, item,
                                  &p->cs, s));
-  if (P > 1) MFP_CHECK(dmalloc(&p->cs_partial, (uint64_t)P * m->rows * vs));
   p->ss = R;
-  p->slices = S;
-  p->L = P;
-  p->device_bytes = m->nnz * (vs + 4 + (p->cs.packed ? 0 : 2)) + ((uint64_t)S + 1) * 4 + (uint64_t)S * P * 4 +
-                    (P > 1 ? (uint64_t)P * m->rows * vs : 0);
+  p->slices = p->cs.S;
+  p->L = p->cs.NI;
+  p->device_bytes = p->cs.device_bytes;
   return MFP_OK;
 }
 
@@ -567,7 +562,7 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
     MFP_CHECK(fill_bytes_zero(y, (size_t)m->rows * dtype_size(dt), s));
     return launch_coo_add(dt, p->ncoo, p->coo_row, p->coo_col, p->coo_val, x, y, s);
   case MFP_SCOO:
-    if (p->colsort) return launch_scoo_cs(dt, m->rows, p->cs, p->scoo_threads, x, y, p->cs_partial, s);
+    if (p->colsort) return launch_scoo_cs(dt, m->rows, p->cs, p->scoo_threads, x, y, s);
     return launch_scoo(dt, m->rows, p->ss, p->L, p->scoo_threads, p->coo_row, p->coo_col,
                        p->coo_val, p->sp, x, y, s);
   case MFP_HYB:
@@ -605,6 +600,7 @@ int mfp_plan_get_info(mfp_plan p, mfp_plan_info *o)
   o->column_bytes = p->off16 ? 2u : 4u;
   o->scoo_colsort = p->colsort ? 1u : 0u;
   o->scoo_packed = p->colsort && p->cs.packed ? 1u : 0u;
+  o->scoo_long_rows = p->colsort ? p->cs.NL : 0u;
   if (p->colsort) o->stored_elements = p->m->nnz;
   return MFP_OK;
 }

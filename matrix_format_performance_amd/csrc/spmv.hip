@@ -565,80 +565,162 @@ __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *_
 }
 
 // ---------------------------------------------------------------------------
-// Column-sorted SCOO: one workgroup per (slice, piece).  The slice's rows
-// accumulate in LDS (ds_add); entries come in column order, so the x reads
-// of a piece sweep one column range and consecutive lanes share x lines.
-// Logical block b = q * S + s keeps equal pieces q of all slices together on
-// one XCD, whose L2 then holds that piece's part of x.  With P = 1 the slice
-// is written to y; otherwise to partial[q][row] and k_scoo_cs_finish adds the
-// P partials of each row in q order.
+// Column-sorted SCOO: one workgroup per work item.  The item's slice rows
+// accumulate in LDS (ds_add); its entries come in column order, so the x
+// reads sweep one column range and consecutive lanes share x lines.  Lanes
+// take 4 consecutive entries at a time with 16-byte loads (two such groups
+// in flight); the unaligned head and tail go one entry per lane.  A single
+// item of an ordinary slice writes y; several write partial slots that
+// k_scoo_cs_finish adds in slot order.  Items of long-row slices add into y
+// (zeroed first) with atomics; ordinary writes skip the long rows.
 // ---------------------------------------------------------------------------
 template <typename T, bool PACKED>
-__global__ void k_scoo_cs(uint32_t rows, uint32_t S, uint32_t P, uint32_t R, uint32_t rbits,
-                          const uint32_t *__restrict__ sp, const uint32_t *__restrict__ pbase,
+struct cs_entry {
+  __device__ __forceinline__ static void decode(uint32_t w, uint32_t r16, uint32_t cbase, uint32_t rbits,
+                                                uint32_t &c, uint32_t &lr)
+  {
+    if constexpr (PACKED) {
+      c = cbase + (w >> rbits);
+      lr = w & ((1u << rbits) - 1u);
+    } else {
+      c = w;
+      lr = r16;
+    }
+  }
+};
+
+__device__ __forceinline__ bool cs_is_long(const uint32_t *__restrict__ mask, uint32_t row)
+{
+  return mask && ((mask[row >> 5] >> (row & 31u)) & 1u);
+}
+
+template <typename T, bool PACKED>
+__global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_reg, uint32_t NL,
+                          const uint4 *__restrict__ items, const uint32_t *__restrict__ ibase,
                           const T *__restrict__ ev, const uint32_t *__restrict__ ec,
-                          const uint16_t *__restrict__ er, const T *__restrict__ x, T *__restrict__ out)
+                          const uint16_t *__restrict__ er, const T *__restrict__ x, T *__restrict__ y,
+                          T *__restrict__ partial, const uint32_t *__restrict__ long_rows,
+                          const uint32_t *__restrict__ long_mask)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *acc = reinterpret_cast<T *>(smem_raw);
-  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t q = b / S, s = b % S;
-  const uint32_t r0 = s * R;
-  const uint32_t lim = min(R, rows - r0);
+  const uint4 it = items[xcd_block(blockIdx.x, gridDim.x)];
+  const uint32_t sl = it.x, e0 = it.y, e1 = it.z, slot = it.w;
+  const uint32_t lim = sl < S_reg ? min(R, rows - sl * R) : min(R, NL - (sl - S_reg) * R);
   const uint32_t bt = blockDim.x, tid = threadIdx.x;
   for (uint32_t i = tid; i < lim; i += bt) acc[i] = (T)0;
   __syncthreads();
-  const uint32_t sb = sp[s], len = sp[s + 1] - sb;
-  const uint32_t e0 = sb + (uint32_t)(((uint64_t)len * q) / P);
-  const uint32_t e1 = sb + (uint32_t)(((uint64_t)len * (q + 1)) / P);
-  const uint32_t cbase = PACKED ? pbase[(uint64_t)s * P + q] : 0u;
-  const uint32_t rmask = (1u << rbits) - 1u;
-  uint32_t i = e0 + tid;
-  for (; i + 3 * bt < e1; i += 4 * bt) {
-    uint32_t c[4], lr[4];
-    T v[4], xv[4];
+  const uint32_t cbase = PACKED ? ibase[xcd_block(blockIdx.x, gridDim.x)] : 0u;
+  // head: up to the first multiple of 4
+  const uint32_t ea = min(e1, (e0 + 3u) & ~3u);
+  for (uint32_t i = e0 + tid; i < ea; i += bt) {
+    uint32_t c, lr;
+    cs_entry<T, PACKED>::decode(ldg_stream(ec + i), PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
+    atomicAdd(acc + lr, ldg_stream(ev + i) * x[c]);
+  }
+  // body: groups of 4 entries, two groups per lane in flight
+  const uint32_t g0 = ea >> 2, g1 = e1 >> 2; // groups [g0, g1) lie inside [ea, e1)
+  uint32_t g = g0 + tid;
+  for (; g + bt < g1; g += 2 * bt) {
+    u32x4 w[2];
+    T v[2][4];
+    uint32_t r16[2][4];
 #pragma unroll
-    for (uint32_t k = 0; k < 4; k++) {
-      const uint32_t w = ldg_stream(ec + i + k * bt);
-      v[k] = ldg_stream(ev + i + k * bt);
-      if constexpr (PACKED) {
-        c[k] = cbase + (w >> rbits);
-        lr[k] = w & rmask;
+    for (uint32_t k = 0; k < 2; k++) {
+      const uint32_t gg = g + k * bt;
+      w[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(ec) + gg);
+      if constexpr (sizeof(T) == 4) {
+        const f32x4 vv = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(ev) + gg);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) v[k][j] = vv[j];
       } else {
-        c[k] = w;
-        lr[k] = ldg_stream(er + i + k * bt);
+        const f64x2 v0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(ev) + 2 * gg);
+        const f64x2 v1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(ev) + 2 * gg + 1);
+        v[k][0] = v0[0];
+        v[k][1] = v0[1];
+        v[k][2] = v1[0];
+        v[k][3] = v1[1];
+      }
+      if constexpr (!PACKED) {
+        const u32x2 rr = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(er) + gg);
+        r16[k][0] = rr[0] & 0xFFFFu;
+        r16[k][1] = rr[0] >> 16;
+        r16[k][2] = rr[1] & 0xFFFFu;
+        r16[k][3] = rr[1] >> 16;
+      } else {
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) r16[k][j] = 0;
       }
     }
+    uint32_t c[2][4], lr[2][4];
+    T xv[2][4];
 #pragma unroll
-    for (uint32_t k = 0; k < 4; k++) xv[k] = x[c[k]];
+    for (uint32_t k = 0; k < 2; k++)
 #pragma unroll
-    for (uint32_t k = 0; k < 4; k++) atomicAdd(acc + lr[k], v[k] * xv[k]);
+      for (uint32_t j = 0; j < 4; j++) cs_entry<T, PACKED>::decode(w[k][j], r16[k][j], cbase, rbits, c[k][j], lr[k][j]);
+#pragma unroll
+    for (uint32_t k = 0; k < 2; k++)
+#pragma unroll
+      for (uint32_t j = 0; j < 4; j++) xv[k][j] = x[c[k][j]];
+#pragma unroll
+    for (uint32_t k = 0; k < 2; k++)
+#pragma unroll
+      for (uint32_t j = 0; j < 4; j++) atomicAdd(acc + lr[k][j], v[k][j] * xv[k][j]);
   }
-  for (; i < e1; i += bt) {
-    const uint32_t w = ldg_stream(ec + i);
-    uint32_t c, lr;
-    if constexpr (PACKED) {
-      c = cbase + (w >> rbits);
-      lr = w & rmask;
-    } else {
-      c = w;
-      lr = ldg_stream(er + i);
+  for (; g < g1; g += bt) {
+    const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(ec) + g);
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+      const uint32_t i = 4 * g + j;
+      uint32_t c, lr;
+      cs_entry<T, PACKED>::decode(w[j], PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
+      atomicAdd(acc + lr, ldg_stream(ev + i) * x[c]);
     }
+  }
+  // tail: entries after the last whole group
+  for (uint32_t i = max(ea, g1 << 2) + tid; i < e1; i += bt) {
+    uint32_t c, lr;
+    cs_entry<T, PACKED>::decode(ldg_stream(ec + i), PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
     atomicAdd(acc + lr, ldg_stream(ev + i) * x[c]);
   }
   __syncthreads();
-  T *o = P == 1 ? out + r0 : out + (uint64_t)q * rows + r0;
-  for (uint32_t lr = tid; lr < lim; lr += bt) o[lr] = acc[lr];
+  if (slot == CS_LONG) {
+    const uint32_t *lrows = long_rows + (uint64_t)(sl - S_reg) * R;
+    for (uint32_t lr = tid; lr < lim; lr += bt) atomicAdd(y + lrows[lr], acc[lr]);
+  } else if (slot == CS_DIRECT) {
+    const uint32_t r0 = sl * R;
+    for (uint32_t lr = tid; lr < lim; lr += bt)
+      if (!cs_is_long(long_mask, r0 + lr)) y[r0 + lr] = acc[lr];
+  } else {
+    T *o = partial + (uint64_t)slot * R;
+    for (uint32_t lr = tid; lr < lim; lr += bt) o[lr] = acc[lr];
+  }
 }
 
 template <typename T>
-__global__ void k_scoo_cs_finish(uint32_t rows, uint32_t P, const T *__restrict__ partial, T *__restrict__ y)
+__global__ void k_scoo_cs_zero_long(uint32_t NL, const uint32_t *__restrict__ long_rows, T *__restrict__ y)
 {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  T d = partial[r];
-  for (uint32_t q = 1; q < P; q++) d = d + partial[(uint64_t)q * rows + r];
-  y[r] = d;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < NL) y[long_rows[j]] = (T)0;
+}
+
+// one workgroup per finish block of CS_FIN_ROWS rows: y[row] = the sum of the
+// slice's slots, in slot order (long rows are left to their atomics)
+template <typename T>
+__global__ void k_scoo_cs_finish(uint32_t rows, uint32_t R, const uint4 *__restrict__ fin,
+                                 const uint32_t *__restrict__ long_mask, const T *__restrict__ partial,
+                                 T *__restrict__ y)
+{
+  const uint4 f = fin[blockIdx.x];
+  const uint32_t sl = f.x, slot0 = f.y, n = f.z;
+  const uint32_t lim = min(R, rows - sl * R);
+  const uint32_t lr = f.w + threadIdx.x;
+  if (lr >= lim || threadIdx.x >= CS_FIN_ROWS) return;
+  const uint32_t row = sl * R + lr;
+  if (cs_is_long(long_mask, row)) return;
+  T d = partial[(uint64_t)slot0 * R + lr];
+  for (uint32_t k = 1; k < n; k++) d = d + partial[(uint64_t)(slot0 + k) * R + lr];
+  y[row] = d;
 }
 
 // ---------------------------------------------------------------------------
@@ -841,28 +923,31 @@ int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t thre
 
 template <typename T, bool PACKED>
 static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
-                     void *partial, hipStream_t s)
+                     hipStream_t s)
 {
   const size_t lds = (size_t)l.R * sizeof(T);
   if (lds > 65536)
     MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T, PACKED>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-  T *out = l.P == 1 ? (T *)y : (T *)partial;
-  k_scoo_cs<T, PACKED><<<l.S * l.P, threads, lds, s>>>(rows, l.S, l.P, l.R, l.rbits, l.sp, l.base,
-                                                       (const T *)l.ev, l.ec, l.er, (const T *)x, out);
+  if (l.NL) {
+    k_scoo_cs_zero_long<T><<<ceil_div_u32(l.NL, 256), 256, 0, s>>>(l.NL, l.long_rows, (T *)y);
+    MFP_CHECK(launched());
+  }
+  k_scoo_cs<T, PACKED><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
+                                                  (const T *)l.ev, l.ec, l.er, (const T *)x, (T *)y,
+                                                  (T *)l.partial, l.long_rows, l.long_mask);
   MFP_CHECK(launched());
-  if (l.P > 1) {
-    k_scoo_cs_finish<T><<<ceil_div_u32(rows, 256), 256, 0, s>>>(rows, l.P, (const T *)partial, (T *)y);
+  if (l.NF) {
+    k_scoo_cs_finish<T><<<l.NF, CS_FIN_ROWS, 0, s>>>(rows, l.R, l.fin, l.long_mask, (const T *)l.partial, (T *)y);
     MFP_CHECK(launched());
   }
   return MFP_OK;
 }
 
 int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
-                   void *partial, hipStream_t s)
+                   hipStream_t s)
 {
-  if (rows == 0) return MFP_OK;
-  if (l.S * (uint64_t)l.P == 0) return MFP_OK;
+  if (rows == 0 || l.NI == 0) return MFP_OK;
   const size_t lds = (size_t)l.R * dtype_size(dtype);
   if (lds > 163840) {
     set_error("column-sorted SCOO slice exceeds the LDS of a workgroup");
@@ -873,10 +958,10 @@ int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t t
     return MFP_ERR_ARG;
   }
   if (dtype == MFP_F32)
-    return l.packed ? scoo_cs_t<float, true>(rows, l, threads, x, y, partial, s)
-                    : scoo_cs_t<float, false>(rows, l, threads, x, y, partial, s);
-  return l.packed ? scoo_cs_t<double, true>(rows, l, threads, x, y, partial, s)
-                  : scoo_cs_t<double, false>(rows, l, threads, x, y, partial, s);
+    return l.packed ? scoo_cs_t<float, true>(rows, l, threads, x, y, s)
+                    : scoo_cs_t<float, false>(rows, l, threads, x, y, s);
+  return l.packed ? scoo_cs_t<double, true>(rows, l, threads, x, y, s)
+                  : scoo_cs_t<double, false>(rows, l, threads, x, y, s);
 }
 
 } // namespace mfp

@@ -279,6 +279,19 @@ def _tuning_dict(t: L.tuning) -> dict:
     return {f: getattr(t, f) for f, _ in L.tuning._fields_}
 
 
+def autotune_candidates(matrix: DeviceMatrix, formats=None):
+    """The (format, tuning) candidates the autotuner would time, untimed."""
+    names = list(FORMATS) if formats is None else list(formats)
+    mask = 0
+    for f in names:
+        mask |= 1 << FORMATS[f]
+    n = C.c_uint32()
+    L.call("mfp_autotune_candidates", matrix.handle, mask, None, 0, C.byref(n))
+    arr = (L.candidate * n.value)()
+    L.call("mfp_autotune_candidates", matrix.handle, mask, arr, n.value, C.byref(n))
+    return [{"format": FORMAT_NAMES[c.format], "tuning": _tuning_dict(c.tuning)} for c in arr]
+
+
 def autotune(matrix: DeviceMatrix, x: DeviceArray, y: DeviceArray, formats=None, reps: int = 3,
              flush: bool = True):
     """Time every candidate (format, tuning) the library sweeps for this

@@ -6,16 +6,19 @@ a distribution over the eight formats.  The network is trained on the GPU by
 the library's own HIP kernels (forward, backward, Adam) against targets made
 from timings this package measures (``sweep``: every autotuner candidate of
 every format, per matrix), and it infers in FP32 or in BF16 on the matrix
-cores.  The pick is the most probable format with its default tuning; the
-exhaustive autotuner stays the fallback and the judge of the pick's regret.
+cores.  The pick is the most probable format; its tuning comes from rules
+fitted on the same timings (per bin of mean row length and row count, the
+tuning with the lowest mean slowdown within the format).  The exhaustive
+autotuner stays the fallback and the judge of the pick's regret.
 
 Command line (``python -m matrix_format_performance_amd.selector``):
 
 * ``sweep --out S.jsonl --count N [--max-nnz M] [--seed K]``: generates
   matrices, records features and candidate timings, one JSON line per matrix;
   rerunning with the same file resumes after the last complete line.
-* ``train --data S.jsonl --out W.npy [--train 1600]``: trains on the first
-  ``--train`` records and reports regret on the rest.
+* ``train --data S.jsonl --out W.npz [--train 1600]``: trains on the first
+  ``--train`` records, fits the tuning rules, saves both (``Selector.load``)
+  and reports regret on the rest.
 """
 from __future__ import annotations
 
@@ -153,6 +156,7 @@ def evaluate(records, probs: np.ndarray) -> dict:
     return {
         "count": len(records),
         "mean_regret": float(np.mean(r_default)),
+        "geomean_slowdown": float(np.exp(np.mean(np.log1p(r_default)))),
         "median_regret": float(np.median(r_default)),
         "p90_regret": float(np.quantile(r_default, 0.9)),
         "within_5pct": float(np.mean(r_default <= 0.05)),
@@ -224,7 +228,44 @@ def sample_spec(index: int, seed: int = 0, max_nnz: int = 1 << 24, min_rows: int
                      seed=index)
 
 
-def measure(matrix: "E.DeviceMatrix", reps: int = 3, flush: bool = True) -> dict:
+def row_length_marks(row_ptr) -> dict:
+    """The row-length marks the autotuner's hybrid widths come from: the
+    rounded-up mean and the 0.75 / 0.9 / 0.99 quantiles (element
+    floor(q (rows - 1)) of the sorted lengths)."""
+    lens = np.diff(np.asarray(row_ptr, np.int64))
+    n = lens.size
+    if n == 0:
+        return {}
+    srt = np.sort(lens)
+    marks = {"avg": int(-(-int(srt.sum()) // n))}
+    for name, q in (("q75", 0.75), ("q90", 0.9), ("q99", 0.99)):
+        marks[name] = int(srt[min(n - 1, int(q * (n - 1)))])
+    return marks
+
+
+def tuning_label(fmt: str, t: dict, marks: dict, dtype=np.float32) -> str:
+    """A name for a candidate's tuning that means the same on every matrix."""
+    if fmt == "csr_vector":
+        return f"lanes{t['lanes']}"
+    if fmt == "csr_adaptive":
+        base = E.DEFAULT_NNZ_PER_WG[np.dtype(dtype)]
+        return {base // 2: "W/2", base: "W", base * 2: "2W"}.get(t["nnz_per_wg"], f"W{t['nnz_per_wg']}")
+    if fmt == "scoo":
+        if t["flags"] & L.MFP_TUNE_SCOO_COLSORT:
+            return f"cs{t['scoo_slice_size']}"
+        return "default" if t["flags"] & L.MFP_TUNE_IS_DEFAULT else f"ss{t['scoo_slice_size']}"
+    if fmt in ("hyb", "hyb_atomic"):
+        w = t["hyb_width"]
+        if w == 0:
+            return "rule"
+        for name in ("avg", "q75", "q90", "q99"):
+            if marks.get(name) == w:
+                return name
+        return f"w{w}"
+    return "default"
+
+
+def measure(matrix: "E.DeviceMatrix", reps: int = 3, flush: bool = True, row_ptr=None) -> dict:
     """Features and exhaustive candidate timings of one device matrix."""
     vt = matrix.vector_dtype
     x = E.DeviceArray.from_host(E.generate_vector(1, matrix.cols, vt))
@@ -235,11 +276,17 @@ def measure(matrix: "E.DeviceMatrix", reps: int = 3, flush: bool = True) -> dict
     t0 = time.perf_counter()
     cands = E.autotune(matrix, x, y, reps=reps, flush=flush)
     t_tune = time.perf_counter() - t0
-    best_ms, default_ms = {}, {}
+    if row_ptr is None:
+        row_ptr = matrix.download()[0]
+    marks = row_length_marks(row_ptr)
+    best_ms, default_ms, listed = {}, {}, []
     for c in cands:
-        if c["status"] != 0 or not math.isfinite(c["ms"]):
-            continue
         f = c["format"]
+        ok = c["status"] == 0 and math.isfinite(c["ms"])
+        listed.append([f, tuning_label(f, c["tuning"], marks, vt), c["ms"] if ok else None,
+                       bool(c["tuning"]["flags"] & L.MFP_TUNE_IS_DEFAULT)])
+        if not ok:
+            continue
         best_ms[f] = min(best_ms.get(f, math.inf), c["ms"])
         if c["tuning"]["flags"] & L.MFP_TUNE_IS_DEFAULT:
             default_ms[f] = c["ms"]
@@ -251,9 +298,116 @@ def measure(matrix: "E.DeviceMatrix", reps: int = 3, flush: bool = True) -> dict
         "default_ms": default_ms,
         "exhaustive_ms": min(best_ms.values()),
         "candidates": len(cands),
+        "cands": listed,
+        "marks": marks,
         "feature_s": t_feat,
         "autotune_s": t_tune,
     }
+
+
+# ---- tuning rules: the tuning of the picked format --------------------------------
+
+def tuning_keys(feats) -> list:
+    """Lookup keys from the most to the least specific: (floor log2(1 + mean
+    row length), floor log2(rows + 1)), then the mean bin alone, then none."""
+    m = int(round(feats[3] * 16 * 1e6)) // 1000000
+    r = int(round(feats[0] * 32 * 1e6)) // 1000000
+    return [f"{m},{r}", f"{m}", ""]
+
+
+def fit_tuning_rules(records, min_count: int = 3) -> dict:
+    """For every format and key, tuning labels ordered by their mean slowdown
+    against the format's best tuning on the training matrices (a label that
+    failed to build counts as 10x slower)."""
+    rules = {}
+    for f in FORMAT_ORDER:
+        agg = {}
+        for r in records:
+            cs = [c for c in r.get("cands", []) if c[0] == f]
+            ok = [c[2] for c in cs if c[2] is not None]
+            if not ok:
+                continue
+            bestf = min(ok)
+            for key in tuning_keys(r["features"]):
+                for c in cs:
+                    agg.setdefault(key, {}).setdefault(c[1], []).append(
+                        c[2] / bestf - 1.0 if c[2] is not None else 10.0)
+        table = {}
+        for key, labels in agg.items():
+            n = max(len(v) for v in labels.values())
+            if n < min_count and key != "":
+                continue
+            table[key] = [lab for lab, _ in sorted(labels.items(), key=lambda kv: (float(np.mean(kv[1])), kv[0]))]
+        rules[f] = table
+    return rules
+
+
+def choose_label(rules: dict, fmt: str, feats, available) -> str:
+    """The best-ranked label of the most specific key that the matrix has."""
+    table = rules.get(fmt, {})
+    for key in tuning_keys(feats):
+        for lab in table.get(key, []):
+            if lab in available:
+                return lab
+    return "default" if "default" in available else sorted(available)[0]
+
+
+class Selector:
+    """The trained network plus the tuning rules: picks (format, tuning) for
+    a matrix from its features, without timing anything."""
+
+    def __init__(self, net: MLP, rules: dict):
+        self.net, self.rules = net, rules
+
+    def save(self, path: str) -> None:
+        np.savez(path, params=self.net.params, rules=np.array(json.dumps(self.rules)))
+
+    @classmethod
+    def load(cls, path: str) -> "Selector":
+        with np.load(path, allow_pickle=False) as z:
+            net = MLP(0)
+            net.params = z["params"]
+            return cls(net, json.loads(str(z["rules"])))
+
+    def rank(self, feats, bf16: bool = True) -> list:
+        probs = self.net.predict(np.asarray(feats, np.float32)[None, :], bf16=bf16)[0]
+        return [FORMAT_ORDER[j] for j in np.argsort(-probs, kind="stable")]
+
+    def pick(self, matrix: "E.DeviceMatrix", bf16: bool = True, row_ptr=None):
+        """-> (format, tuning dict) for a device matrix."""
+        feats = features(matrix)
+        if row_ptr is None:
+            row_ptr = matrix.download()[0]
+        marks = row_length_marks(row_ptr)
+        for fmt in self.rank(feats, bf16):
+            cands = E.autotune_candidates(matrix, [fmt])
+            if not cands:
+                continue
+            labelled = {tuning_label(fmt, c["tuning"], marks, matrix.vector_dtype): c["tuning"] for c in cands}
+            return fmt, labelled[choose_label(self.rules, fmt, feats, labelled)]
+        raise RuntimeError("no format has candidates for this matrix")
+
+
+def record_time(record: dict, fmt: str, label: str):
+    for c in record.get("cands", []):
+        if c[0] == fmt and c[1] == label:
+            return c[2] if c[2] is not None else math.inf
+    return math.inf
+
+
+def evaluate_with_rules(records, probs: np.ndarray, rules: dict) -> dict:
+    """Regret of the network's format with the rules' tuning."""
+    pick = np.argmax(probs, axis=1)
+    best_t = np.array([r["exhaustive_ms"] for r in records])
+    t = []
+    for r, j in zip(records, pick):
+        f = FORMAT_ORDER[j]
+        avail = {c[1] for c in r["cands"] if c[0] == f}
+        t.append(record_time(r, f, choose_label(rules, f, r["features"], avail)) if avail else math.inf)
+    rg = regret(np.array(t), best_t)
+    return {"mean_regret": float(np.mean(rg)), "median_regret": float(np.median(rg)),
+            "geomean_slowdown": float(np.exp(np.mean(np.log1p(rg)))), "p90_regret": float(np.quantile(rg, 0.9)),
+            "within_5pct": float(np.mean(rg <= 0.05))}
 
 
 def _done_indices(path: str) -> set:
@@ -287,7 +441,7 @@ def sweep(out: str, count: int, seed: int = 0, max_nnz: int = 1 << 24, reps: int
             rp, ci, va = E.generate(spec, np.float32)
             cols = {"stencil2d": spec.n * spec.n, "stencil3d": spec.n ** 3}.get(spec.family, rp.size - 1)
             A = E.DeviceMatrix.from_csr(rp, ci, va, cols)
-            rec = measure(A, reps=reps)
+            rec = measure(A, reps=reps, row_ptr=rp)
             A.destroy()
             rec.update({"index": i, "family": spec.family, "rows": int(rp.size - 1), "cols": int(cols),
                         "nnz": int(ci.size), "spec": {k: getattr(spec, k) for k in
@@ -336,6 +490,7 @@ def main(argv=None) -> int:
     t.add_argument("--epochs", type=int, default=300)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--report", default=None, help="also write the evaluation JSON here")
     a = ap.parse_args(argv)
     E.init()
     if a.cmd == "sweep":
@@ -347,13 +502,19 @@ def main(argv=None) -> int:
     tr, te = recs[:a.train], recs[a.train:]
     X, P = dataset(tr)
     net = train(X, P, epochs=a.epochs, lr=a.lr, seed=a.seed, log=lambda m: print(m, flush=True))
-    np.save(a.out, net.params)
-    rep = {"train": evaluate(tr, net.predict(X))}
+    rules = fit_tuning_rules(tr)
+    Selector(net, rules).save(a.out)
+    rep = {"train": evaluate(tr, net.predict(X)), "train_with_rules": evaluate_with_rules(tr, net.predict(X), rules)}
     if te:
         Xt, _ = dataset(te)
-        rep["held_out"] = evaluate(te, net.predict(Xt))
-        rep["held_out_bf16"] = evaluate(te, net.predict(Xt, bf16=True))
+        for name, bf in (("held_out", False), ("held_out_bf16", True)):
+            q = net.predict(Xt, bf16=bf)
+            rep[name] = evaluate(te, q)
+            rep[name + "_with_rules"] = evaluate_with_rules(te, q, rules)
     print(json.dumps(rep, indent=1))
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(rep, f, indent=1)
     return 0
 
 

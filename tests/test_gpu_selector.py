@@ -8,6 +8,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle
 from gpu_cases import announce, case
 from oracle import selector as S
 
@@ -135,12 +136,15 @@ def test_selector_held_out_regret(gpu, tmp_path):
     Xt, _ = sel.dataset(te)
     rep = sel.evaluate(te, net.predict(Xt))
     rep_bf16 = sel.evaluate(te, net.predict(Xt, bf16=True))
+    rules = sel.fit_tuning_rules(tr)
+    rep_rules = sel.evaluate_with_rules(te, net.predict(Xt, bf16=True), rules)
     fixed_train = sel.evaluate(tr, net.predict(X))["best_fixed_format"]
     j = sel.FORMAT_ORDER.index(fixed_train)
     best_t = np.array([r["exhaustive_ms"] for r in te])
     t_fixed = np.array([sel.format_times(r, "default")[j] for r in te])
     fixed_regret = float(np.mean(sel.regret(t_fixed, best_t)))
-    summary = {"held_out": rep, "held_out_bf16": rep_bf16, "fixed_format": fixed_train,
+    summary = {"held_out": rep, "held_out_bf16": rep_bf16, "held_out_bf16_with_rules": rep_rules,
+               "fixed_format": fixed_train,
                "fixed_format_held_out_mean_regret": fixed_regret,
                "autotune_s_mean": float(np.mean([r["autotune_s"] for r in recs])),
                "feature_s_mean": float(np.mean([r["feature_s"] for r in recs]))}
@@ -152,4 +156,20 @@ def test_selector_held_out_regret(gpu, tmp_path):
     # what the network gains over the method without it (one fixed format)
     assert rep["mean_regret"] < fixed_regret
     assert rep_bf16["mean_regret"] < fixed_regret
+    assert rep_rules["mean_regret"] < fixed_regret
+    # the saved selector picks a (format, tuning) that builds and multiplies correctly
+    path_sel = str(tmp_path / "sel.npz")
+    sel.Selector(net, rules).save(path_sel)
+    loaded = sel.Selector.load(path_sel)
+    assert np.array_equal(loaded.net.params, net.params)
+    rp, ci, va, cols = case("powerlaw_20k", "float32")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    fmt, tuning = loaded.pick(A)
+    assert fmt in sel.FORMAT_ORDER
+    x = gpu.generate_vector(2, cols, np.float32)
+    xd, yd = gpu.DeviceArray.from_host(x), gpu.DeviceArray(rp.size - 1, np.float32)
+    gpu.Plan(A, fmt, **tuning).spmv(xd, yd)
+    gpu.synchronize()
+    y64, absdot = oracle.spmv_f64(rp, ci, va, x)
+    assert oracle.tolerance_check(yd.to_host(), y64, absdot, 1e-5) <= 1e-5
     net.destroy()

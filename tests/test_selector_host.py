@@ -1,0 +1,64 @@
+"""Host logic of the format selector (no GPU): targets, regret, the
+row-length marks behind the hybrid widths, tuning labels and rules."""
+import math
+
+import numpy as np
+import pytest
+
+from matrix_format_performance_amd import selector as sel
+
+
+def test_soft_targets_and_regret():
+    t = np.array([[2.0, 1.0, math.inf, 1.05, 4, 4, 4, 4]])
+    p = sel.soft_targets(t, tau=0.05)
+    assert p.shape == (1, 8) and p[0, 2] == 0 and abs(p.sum() - 1) < 1e-6
+    assert p[0, 1] / p[0, 3] == pytest.approx(math.e, rel=1e-5)  # 5% slower = one tau
+    assert np.allclose(sel.regret([1.1, 2.0], [1.0, 2.0]), [0.1, 0.0])
+
+
+def test_row_length_marks_follow_the_autotuner():
+    # lengths 0..99 shuffled: ceil mean 50 (4950 / 100 rounded up), quantiles at floor(q * 99)
+    rng = np.random.default_rng(0)
+    lens = rng.permutation(100)
+    rp = np.concatenate([[0], np.cumsum(lens)])
+    m = sel.row_length_marks(rp)
+    assert m == {"avg": 50, "q75": 74, "q90": 89, "q99": 98}
+
+
+def test_tuning_labels():
+    base = {"lanes": 8, "nnz_per_wg": 2040, "hyb_width": 0, "scoo_slice_size": 512, "flags": 0}
+    marks = {"avg": 5, "q75": 7, "q90": 7, "q99": 30}
+    assert sel.tuning_label("csr_vector", base, marks) == "lanes8"
+    assert sel.tuning_label("csr_adaptive", dict(base, nnz_per_wg=1020), marks) == "W/2"
+    assert sel.tuning_label("hyb", base, marks) == "rule"
+    assert sel.tuning_label("hyb", dict(base, hyb_width=7), marks) == "q75"  # first matching mark
+    assert sel.tuning_label("scoo", dict(base, flags=4, scoo_slice_size=16384), marks) == "cs16384"
+    assert sel.tuning_label("scoo", dict(base, flags=2), marks) == "default"
+    assert sel.tuning_label("ell", base, marks) == "default"
+
+
+def _rec(mean_bin_feat, rows_feat, cands):
+    f = [0.0] * 64
+    f[3], f[0] = mean_bin_feat, rows_feat
+    best = min(c[2] for c in cands if c[2] is not None)
+    return {"features": f, "cands": cands, "exhaustive_ms": best}
+
+
+def test_rules_pick_the_label_with_the_lowest_mean_slowdown_per_key():
+    # mean bin 2 (log2(1+mean) in [2,3)): lanes4 best; mean bin 5: lanes32 best
+    recs = []
+    for k in range(4):
+        recs.append(_rec(2.5 / 16, 10.2 / 32, [["csr_vector", "lanes4", 1.0, True], ["csr_vector", "lanes32", 1.5, False]]))
+        recs.append(_rec(5.5 / 16, 12.2 / 32, [["csr_vector", "lanes4", 2.0, True], ["csr_vector", "lanes32", 1.0, False]]))
+    rules = sel.fit_tuning_rules(recs)
+    assert rules["csr_vector"]["2,10"][0] == "lanes4"
+    assert rules["csr_vector"]["5,12"][0] == "lanes32"
+    f = [0.0] * 64
+    f[3], f[0] = 5.2 / 16, 20.5 / 32  # unseen row bin: falls back to the mean bin
+    assert sel.choose_label(rules, "csr_vector", f, {"lanes4", "lanes32"}) == "lanes32"
+    # a label the matrix lacks is skipped
+    assert sel.choose_label(rules, "csr_vector", f, {"lanes4"}) == "lanes4"
+    probs = np.zeros((2, 8))
+    probs[:, sel.FORMAT_ORDER.index("csr_vector")] = 1
+    ev = sel.evaluate_with_rules(recs[:2], probs, rules)
+    assert ev["mean_regret"] == 0 and ev["within_5pct"] == 1
