@@ -254,6 +254,30 @@ def main():
     log(f"[bench] exhaustive selection over {len(cands)} candidates in {t_sel:.1f}s: {best} "
         f"{tuning_summary(best, best_c['tuning'])}")
 
+    # the learned pick (features + network + tuning rules, nothing timed)
+    # against the exhaustive best
+    from matrix_format_performance_amd import selector as sel_mod
+    selector_out = None
+    if os.path.exists(sel_mod.DEFAULT_WEIGHTS):
+        t_pick = time.time()
+        sl = sel_mod.Selector.default()
+        pfmt, ptun = sl.pick(A, row_ptr=rp)
+        t_pick = time.time() - t_pick
+        pc = next((c for c in ok if c["format"] == pfmt and c["tuning"] == ptun), None)
+        if pc is None:
+            pp = mfp.Plan(A, pfmt, **ptun)
+            time_plan(mfp, pp, xd, yd, 1, flush, timer)
+            pms = statistics.median(time_plan(mfp, pp, xd, yd, args.select_reps, flush, timer))
+            pp.destroy()
+        else:
+            pms = pc["ms"]
+        selector_out = {"pick": pfmt, "tuning": tuning_summary(pfmt, ptun), "ms": round(pms, 5),
+                        "slowdown_vs_exhaustive": round(pms / best_c["ms"], 4),
+                        "pick_seconds": round(t_pick, 4), "exhaustive_seconds": round(t_sel, 3)}
+        sl.net.destroy()
+        log(f"[bench] learned pick {pfmt} {selector_out['tuning']}: {pms * 1e3:.1f} us "
+            f"({selector_out['slowdown_vs_exhaustive']}x the exhaustive best)")
+
     # correctness of the measured format on this very workload
     time_plan(mfp, plan, xd, yd, 1, flush, timer)
     mfp.synchronize()
@@ -310,6 +334,7 @@ def main():
             "frac_of_measured_copy": round(value / copy_gbs, 4),
         },
         "pct_hbm_roofline": round(100 * value / HBM_PEAK_GBS, 2),
+        "learned_pick": selector_out,
         "formats": table,
         "roofline": {
             "bound": "hbm",

@@ -36,6 +36,7 @@ from . import _lib as L
 from . import engine as E
 
 NI, NH, NO = 64, 128, 8
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_mi355x.npz")
 FORMAT_ORDER = [E.FORMAT_NAMES[i] for i in range(NO)]
 
 
@@ -363,6 +364,12 @@ class Selector:
         np.savez(path, params=self.net.params, rules=np.array(json.dumps(self.rules)))
 
     @classmethod
+    def default(cls) -> "Selector":
+        """The selector trained on this package's 2,000-matrix MI355X sweep
+        (selector_mi355x.npz next to this file)."""
+        return cls.load(DEFAULT_WEIGHTS)
+
+    @classmethod
     def load(cls, path: str) -> "Selector":
         with np.load(path, allow_pickle=False) as z:
             net = MLP(0)
@@ -466,10 +473,48 @@ def load_records(path: str) -> list:
     return [recs[k] for k in sorted(recs)]
 
 
-def dataset(records):
+def rule_times(records, rules) -> np.ndarray:
+    """Per-format time with the tuning the rules choose (inf where none)."""
+    out = np.full((len(records), NO), math.inf)
+    for i, r in enumerate(records):
+        for j, f in enumerate(FORMAT_ORDER):
+            avail = {c[1] for c in r.get("cands", []) if c[0] == f}
+            if avail:
+                out[i, j] = record_time(r, f, choose_label(rules, f, r["features"], avail))
+    return out
+
+
+def dataset(records, rules=None, tau: float = 0.05):
+    """Features and target distributions: from each format's best time, or,
+    given rules, from the time of the tuning the rules pick (what a pick of
+    that format would get)."""
     X = np.array([r["features"] for r in records], np.float32)
-    T = np.stack([format_times(r, "best") for r in records])
-    return X, soft_targets(T)
+    T = rule_times(records, rules) if rules is not None else np.stack([format_times(r, "best") for r in records])
+    return X, soft_targets(T, tau)
+
+
+def fixed_format_baseline(train_records, test_records, rules=None) -> dict:
+    """The method without a selector: the one format with the lowest mean
+    regret on the training matrices (default tuning, or the rules' tuning),
+    applied to every test matrix."""
+    def times(recs):
+        return rule_times(recs, rules) if rules is not None else np.stack([format_times(r, "default") for r in recs])
+    btr = np.array([r["exhaustive_ms"] for r in train_records])
+    ttr = times(train_records)
+    means = [float(np.mean(ttr[:, j] / btr - 1)) if np.all(np.isfinite(ttr[:, j])) else math.inf for j in range(NO)]
+    j = int(np.argmin(means))
+    bte = np.array([r["exhaustive_ms"] for r in test_records])
+    rg = times(test_records)[:, j] / bte - 1
+    return {"format": FORMAT_ORDER[j], "mean_regret": float(np.mean(rg)),
+            "geomean_slowdown": float(np.exp(np.mean(np.log1p(rg)))), "within_5pct": float(np.mean(rg <= 0.05))}
+
+
+def train_selector(records, tau=0.05, epochs=300, lr=2e-3, batch=128, seed=0, log=None):
+    """Rules from the records, then the network on rule-based targets."""
+    rules = fit_tuning_rules(records)
+    X, P = dataset(records, rules, tau)
+    net = train(X, P, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
+    return Selector(net, rules)
 
 
 def main(argv=None) -> int:
@@ -491,6 +536,8 @@ def main(argv=None) -> int:
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--report", default=None, help="also write the evaluation JSON here")
+    t.add_argument("--tau", type=float, default=0.05, help="target temperature (relative slowdown)")
+    t.add_argument("--tune", action="store_true", help="choose tau and epochs on a validation split")
     a = ap.parse_args(argv)
     E.init()
     if a.cmd == "sweep":
@@ -500,17 +547,45 @@ def main(argv=None) -> int:
         return 0
     recs = load_records(a.data)
     tr, te = recs[:a.train], recs[a.train:]
-    X, P = dataset(tr)
-    net = train(X, P, epochs=a.epochs, lr=a.lr, seed=a.seed, log=lambda m: print(m, flush=True))
-    rules = fit_tuning_rules(tr)
-    Selector(net, rules).save(a.out)
-    rep = {"train": evaluate(tr, net.predict(X)), "train_with_rules": evaluate_with_rules(tr, net.predict(X), rules)}
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    # hyper-parameters on a validation split of the training matrices
+    grid = [(tau, ep) for tau in (0.02, 0.05, 0.1, 0.2) for ep in (150, 300, 600)] if a.tune else [(a.tau, a.epochs)]
+    choice, tuning_log = grid[0], []
+    if len(grid) > 1:
+        nv = len(tr) // 5
+        fit, val = tr[:-nv], tr[-nv:]
+        best = math.inf
+        for tau, ep in grid:
+            sl = train_selector(fit, tau, ep, a.lr, seed=a.seed)
+            Xv = np.array([r["features"] for r in val], np.float32)
+            ev = evaluate_with_rules(val, sl.net.predict(Xv, bf16=True), sl.rules)
+            sl.net.destroy()
+            tuning_log.append({"tau": tau, "epochs": ep, **ev})
+            log(f"tau={tau} epochs={ep}: validation mean regret {ev['mean_regret']:.4f}")
+            if ev["mean_regret"] < best:
+                best, choice = ev["mean_regret"], (tau, ep)
+    tau, ep = choice
+    sl = train_selector(tr, tau, ep, a.lr, seed=a.seed, log=log)
+    sl.save(a.out)
+    Xtr = np.array([r["features"] for r in tr], np.float32)
+    rep = {"config": {"tau": tau, "epochs": ep, "lr": a.lr, "train": len(tr), "held_out": len(te)},
+           "validation": tuning_log,
+           "train_with_rules": evaluate_with_rules(tr, sl.net.predict(Xtr, bf16=True), sl.rules)}
     if te:
-        Xt, _ = dataset(te)
-        for name, bf in (("held_out", False), ("held_out_bf16", True)):
-            q = net.predict(Xt, bf16=bf)
-            rep[name] = evaluate(te, q)
-            rep[name + "_with_rules"] = evaluate_with_rules(te, q, rules)
+        Xt = np.array([r["features"] for r in te], np.float32)
+        for name, bf in (("held_out_fp32", False), ("held_out_bf16", True)):
+            q = sl.net.predict(Xt, bf16=bf)
+            rep[name] = evaluate_with_rules(te, q, sl.rules)
+            rep[name]["format_accuracy"] = evaluate(te, q)["format_accuracy"]
+        rep["held_out_default_tuning"] = evaluate(te, sl.net.predict(Xt, bf16=True))
+        rep["no_selector_fixed_format_default_tuning"] = fixed_format_baseline(tr, te)
+        rep["no_selector_fixed_format_rules_tuning"] = fixed_format_baseline(tr, te, sl.rules)
+        rep["cost_s"] = {"features_mean": float(np.mean([r["feature_s"] for r in te])),
+                         "exhaustive_autotune_mean": float(np.mean([r["autotune_s"] for r in te]))}
+        t0 = time.perf_counter()
+        for _ in range(20):
+            sl.net.predict(Xt[:1], bf16=True)
+        rep["cost_s"]["inference_one_matrix"] = (time.perf_counter() - t0) / 20
     print(json.dumps(rep, indent=1))
     if a.report:
         with open(a.report, "w") as f:
