@@ -85,6 +85,7 @@ int launch_csr_vector_lp(int dt, uint32_t lanes, uint32_t rows, const uint32_t *
 int launch_ell_lp(int dt, bool off16, uint32_t rows, uint32_t w, const void *cols, const uint32_t *base,
                   const void *ev, const void *x, void *y, hipStream_t s);
 int row_span_max(uint32_t rows, const uint32_t *rp, const uint32_t *ci, uint32_t *out, hipStream_t s);
+uint32_t ell_lp_stride(uint32_t rows);
 int ell_lp_build(uint32_t rows, const uint32_t *rp, const uint32_t *ci, const void *a, uint32_t w,
                  bool off16, void *ev, void *cols, uint32_t *base, hipStream_t s);
 int to_lp_values(const void *in, int din, void *out, int dout, uint64_t n, hipStream_t s);
@@ -181,16 +182,19 @@ static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipS
   uint32_t span = 0;
   MFP_CHECK(row_span_max(m->rows, m->row_ptr, m->columns, &span, s));
   p->off16 = span < 65536u && !(t.flags & MFP_TUNE_NO_OFFSET16);
-  const uint64_t slots = (uint64_t)m->rows * w;
-  const uint64_t need = slots * (2 + (p->off16 ? 2 : 4)) + (p->off16 ? (uint64_t)m->rows * 4 : 0);
+  const uint32_t stride = ell_lp_stride(m->rows); // rows rounded up to a multiple of 64
+  // with 16-bit offsets the slots are packed 4 per 16-byte word (w rounded up to 4)
+  const uint64_t slots = (uint64_t)stride * (p->off16 ? (w + 3) / 4 * 4 : w);
+  const uint64_t need = slots * (p->off16 ? 4 : 6) + (p->off16 ? (uint64_t)stride * 4 : 0);
   if (need > budget_of(t)) {
     set_error("ELL part needs " + std::to_string(need) + " bytes, above the budget");
     return MFP_ERR_BUDGET;
   }
   p->ell_w = w;
-  MFP_CHECK(dmalloc(&p->ell_val, slots * 2));
-  MFP_CHECK(dmalloc(&p->lp_cols, slots * (p->off16 ? 2 : 4)));
-  if (p->off16) MFP_CHECK(dmalloc((void **)&p->lp_base, (uint64_t)m->rows * 4));
+  // with 16-bit offsets, value and offset share one u32 slot in lp_cols
+  if (!p->off16) MFP_CHECK(dmalloc(&p->ell_val, slots * 2));
+  MFP_CHECK(dmalloc(&p->lp_cols, slots * 4));
+  if (p->off16) MFP_CHECK(dmalloc((void **)&p->lp_base, (uint64_t)stride * 4));
   MFP_CHECK(ell_lp_build(m->rows, m->row_ptr, m->columns, m->values, w, p->off16, p->ell_val,
                          p->lp_cols, p->lp_base, s));
   MFP_HIP(hipStreamSynchronize(s));
@@ -609,6 +613,8 @@ int mfp_plan_array(mfp_plan p, int which, const void **ptr, uint64_t *count)
 {
   MFP_ARG(p && ptr && count, "null pointer");
   const uint64_t slots = (uint64_t)p->m->rows * p->ell_w;
+  MFP_ARG(!(is_lp(p->dtype) && (which == MFP_ARR_ELL_VALUES || which == MFP_ARR_ELL_COLS)),
+          "16-bit-value ELL plans keep their own packed layout");
   switch (which) {
   case MFP_ARR_ELL_VALUES: *ptr = p->ell_val; *count = slots; break;
   case MFP_ARR_ELL_COLS: *ptr = p->ell_col; *count = slots; break;

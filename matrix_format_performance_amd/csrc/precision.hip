@@ -209,37 +209,104 @@ __global__ __launch_bounds__(256) void k_csr_vector_lp(uint32_t rows, const uint
   if (lane == 0 && row < rows) y[row] = acc.s + acc.c;
 }
 
-// ---- low-precision ELL (thread per row, Dot2, optional 16-bit column offsets) -------------
+// ---- low-precision ELL (Dot2, optional 16-bit column offsets) -------------------------------
+// Sliced layout: the w slots of each block of 64 rows are contiguous, slot k
+// of row r at (r / 64 * w + k) * 64 + r % 64, so a wave (one block) reads one
+// contiguous run instead of w strided ones.  With 16-bit offsets a slot is
+// one u32, offset << 16 | value bits; otherwise values (u16) and columns
+// (u32) are separate arrays.  Thread per row; each row adds its slots in
+// order.
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+
 template <int DT, bool OFF16>
-__global__ __launch_bounds__(256) void k_ell_lp(uint32_t rows, uint32_t w, const void *__restrict__ cols,
-                                                const uint32_t *__restrict__ base,
-                                                const uint16_t *__restrict__ ev,
-                                                const float *__restrict__ x, float *__restrict__ y)
+struct ell_lp_row {
+  const void *cols;
+  const uint16_t *ev;
+  const float *x;
+  uint32_t r, w, b0;
+  __device__ __forceinline__ uint64_t at(uint32_t k) const { return ((uint64_t)(r >> 6) * w + k) * 64u + (r & 63u); }
+  // N slots from k: every load issued before the first use
+  template <uint32_t N> __device__ __forceinline__ void group(uint32_t k, comp &acc) const
+  {
+    uint32_t a[N], c[N];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++) {
+      a[q] = ldg_stream(ev + at(k + q));
+      c[q] = ldg_stream((const uint32_t *)cols + at(k + q));
+    }
+    float xv[N];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++) xv[q] = x[c[q]];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++) dot2_add(acc, lp_val<DT>((uint16_t)a[q]), xv[q]);
+  }
+};
+
+// 16-bit offsets: packed slots, 4 consecutive slots of a row in one 16-byte
+// word; word j of row r at ((r / 64) * w4 + j) * 64 + r % 64 (w4 = w / 4
+// rounded up; padding slots hold value 0, offset 0).
+template <int DT>
+struct ell_lp_row16 {
+  const u32x4v *slots;
+  const float *x;
+  uint32_t r, w4, b0;
+  template <uint32_t N> __device__ __forceinline__ void group(uint32_t j, comp &acc) const
+  {
+    u32x4v v[N];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++)
+      v[q] = __builtin_nontemporal_load(slots + ((uint64_t)(r >> 6) * w4 + j + q) * 64u + (r & 63u));
+    float xv[N][4];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++)
+#pragma unroll
+      for (uint32_t e = 0; e < 4; e++) xv[q][e] = x[b0 + (v[q][e] >> 16)];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++)
+#pragma unroll
+      for (uint32_t e = 0; e < 4; e++) dot2_add(acc, lp_val<DT>((uint16_t)(v[q][e] & 0xFFFFu)), xv[q][e]);
+  }
+};
+
+template <int DT, bool OFF16>
+__global__ __launch_bounds__(256) void k_ell_lp(uint32_t rows, uint32_t stride, uint32_t w,
+                                                const void *__restrict__ cols, const uint32_t *__restrict__ base,
+                                                const uint16_t *__restrict__ ev, const float *__restrict__ x,
+                                                float *__restrict__ y)
 {
   const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
   if (r >= rows) return;
+  (void)stride;
   comp acc{0.f, 0.f};
-  const uint32_t b0 = OFF16 ? base[r] : 0u;
-  uint64_t o = r;
-  uint32_t k = 0;
-  auto col = [&](uint64_t idx) -> uint32_t {
-    if constexpr (OFF16)
-      return b0 + (uint32_t)ldg_stream((const uint16_t *)cols + idx);
-    else
-      return ldg_stream((const uint32_t *)cols + idx);
-  };
-  for (; k + 4 <= w; k += 4) {
-    const uint16_t a0 = ldg_stream(ev + o), a1 = ldg_stream(ev + o + rows);
-    const uint16_t a2 = ldg_stream(ev + o + 2ull * rows), a3 = ldg_stream(ev + o + 3ull * rows);
-    const uint32_t c0 = col(o), c1 = col(o + rows), c2 = col(o + 2ull * rows), c3 = col(o + 3ull * rows);
-    const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-    dot2_add(acc, lp_val<DT>(a0), x0);
-    dot2_add(acc, lp_val<DT>(a1), x1);
-    dot2_add(acc, lp_val<DT>(a2), x2);
-    dot2_add(acc, lp_val<DT>(a3), x3);
-    o += 4ull * rows;
+  // groups of slots with no guarded loads, so each group's loads are in flight together
+  if constexpr (OFF16) {
+    const uint32_t w4 = (w + 3) / 4;
+    const ell_lp_row16<DT> row{(const u32x4v *)cols, x, r, w4, base[r]};
+    uint32_t j = 0;
+    for (; j + 3 <= w4; j += 3) row.template group<3>(j, acc);
+    if (j + 2 <= w4) {
+      row.template group<2>(j, acc);
+      j += 2;
+    }
+    if (j < w4) row.template group<1>(j, acc);
+  } else {
+    const ell_lp_row<DT, false> row{cols, ev, x, r, w, 0u};
+    uint32_t k = 0;
+    for (; k + 12 <= w; k += 12) row.template group<12>(k, acc);
+    if (k + 8 <= w) {
+      row.template group<8>(k, acc);
+      k += 8;
+    }
+    if (k + 4 <= w) {
+      row.template group<4>(k, acc);
+      k += 4;
+    }
+    if (k + 2 <= w) {
+      row.template group<2>(k, acc);
+      k += 2;
+    }
+    if (k < w) row.template group<1>(k, acc);
   }
-  for (; k < w; k++, o += rows) dot2_add(acc, lp_val<DT>(ldg_stream(ev + o)), x[col(o)]);
   y[r] = acc.s + acc.c;
 }
 
@@ -258,24 +325,29 @@ __global__ void k_row_span(uint32_t rows, const uint32_t *__restrict__ rp, const
 }
 
 // ELL build for 16-bit values: column-major, padding (0, offset 0 / col 0).
-__global__ void k_ell_lp_build(uint32_t rows, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
-                               const uint16_t *__restrict__ a, uint32_t w, bool off16,
-                               uint16_t *__restrict__ ev, void *cols, uint32_t *__restrict__ base)
+__global__ void k_ell_lp_build(uint32_t rows, uint32_t stride, const uint32_t *__restrict__ rp,
+                               const uint32_t *__restrict__ ci, const uint16_t *__restrict__ a, uint32_t w,
+                               bool off16, uint16_t *__restrict__ ev, void *cols, uint32_t *__restrict__ base)
 {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  const uint32_t s = rp[r], len = rp[r + 1] - s;
+  if (r >= stride) return;
+  const uint32_t s = r < rows ? rp[r] : 0u, len = r < rows ? rp[r + 1] - s : 0u;
   const uint32_t b0 = len ? ci[s] : 0u;
   if (off16) base[r] = b0;
-  for (uint32_t k = 0; k < w; k++) {
-    const uint64_t o = (uint64_t)r + (uint64_t)k * rows;
+  const uint32_t kend = off16 ? (w + 3) / 4 * 4 : w; // packed words also hold the padding slots
+  for (uint32_t k = 0; k < kend; k++) {
     const bool in = k < len;
-    ev[o] = in ? a[s + k] : (uint16_t)0;
+    const uint16_t v = in ? a[s + k] : (uint16_t)0;
     const uint32_t c = in ? ci[s + k] : 0u;
-    if (off16)
-      ((uint16_t *)cols)[o] = in ? (uint16_t)(c - b0) : (uint16_t)0;
-    else
+    if (off16) {
+      const uint32_t w4 = (w + 3) / 4;
+      const uint64_t o = (((uint64_t)(r >> 6) * w4 + k / 4) * 64u + (r & 63u)) * 4u + k % 4;
+      ((uint32_t *)cols)[o] = (in ? (c - b0) << 16 : 0u) | v;
+    } else {
+      const uint64_t o = ((uint64_t)(r >> 6) * w + k) * 64u + (r & 63u);
+      ev[o] = v;
       ((uint32_t *)cols)[o] = c;
+    }
   }
 }
 
@@ -323,20 +395,23 @@ int launch_csr_vector_lp(int dt, uint32_t lanes, uint32_t rows, const uint32_t *
   return launched_lp();
 }
 
+uint32_t ell_lp_stride(uint32_t rows) { return (uint32_t)(((uint64_t)rows + 63) / 64 * 64); }
+
 int launch_ell_lp(int dt, bool off16, uint32_t rows, uint32_t w, const void *cols, const uint32_t *base,
                   const void *ev, const void *x, void *y, hipStream_t s)
 {
   if (rows == 0) return MFP_OK;
+  const uint32_t stride = ell_lp_stride(rows);
   const uint32_t grid = ceil_div_u32(rows, 256);
   const uint16_t *e = (const uint16_t *)ev;
   const float *xv = (const float *)x;
   float *yv = (float *)y;
   if (dt == MFP_BF16) {
-    if (off16) k_ell_lp<MFP_BF16, true><<<grid, 256, 0, s>>>(rows, w, cols, base, e, xv, yv);
-    else k_ell_lp<MFP_BF16, false><<<grid, 256, 0, s>>>(rows, w, cols, base, e, xv, yv);
+    if (off16) k_ell_lp<MFP_BF16, true><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
+    else k_ell_lp<MFP_BF16, false><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
   } else {
-    if (off16) k_ell_lp<MFP_F16, true><<<grid, 256, 0, s>>>(rows, w, cols, base, e, xv, yv);
-    else k_ell_lp<MFP_F16, false><<<grid, 256, 0, s>>>(rows, w, cols, base, e, xv, yv);
+    if (off16) k_ell_lp<MFP_F16, true><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
+    else k_ell_lp<MFP_F16, false><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
   }
   return launched_lp();
 }
@@ -357,8 +432,9 @@ int ell_lp_build(uint32_t rows, const uint32_t *rp, const uint32_t *ci, const vo
                  bool off16, void *ev, void *cols, uint32_t *base, hipStream_t s)
 {
   if (rows == 0 || w == 0) return MFP_OK;
-  k_ell_lp_build<<<ceil_div_u32(rows, 256), 256, 0, s>>>(rows, rp, ci, (const uint16_t *)a, w, off16,
-                                                         (uint16_t *)ev, cols, base);
+  const uint32_t stride = ell_lp_stride(rows);
+  k_ell_lp_build<<<ceil_div_u32(stride, 256), 256, 0, s>>>(rows, stride, rp, ci, (const uint16_t *)a, w, off16,
+                                                           (uint16_t *)ev, cols, base);
   return launched_lp();
 }
 

@@ -17,6 +17,8 @@
 #include "mfp_internal.h"
 #include "wave64.h"
 
+#include <type_traits>
+
 namespace mfp {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -404,20 +406,35 @@ __device__ __forceinline__ void ell_row(uint32_t r, uint32_t rows, uint32_t w,
 {
   T d = 0;
   uint64_t o = r;
+  // groups of 8 slots, then 4 / 2 / 1: each group's loads are all in flight
+  // before the first fma, and the fma order stays slot order
+  auto group = [&](auto n_const) {
+    constexpr uint32_t N = decltype(n_const)::value;
+    T a[N];
+    uint32_t c[N];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++) {
+      a[q] = ldg_stream(ev + o + (uint64_t)q * rows);
+      c[q] = ldg_stream(ec + o + (uint64_t)q * rows);
+    }
+    T xv[N];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++) xv[q] = x[c[q]];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++) d = fma_t(a[q], xv[q], d);
+    o += (uint64_t)N * rows;
+  };
   uint32_t k = 0;
-  for (; k + 4 <= w; k += 4) {
-    const T a0 = ldg_stream(ev + o), a1 = ldg_stream(ev + o + rows);
-    const T a2 = ldg_stream(ev + o + 2ull * rows), a3 = ldg_stream(ev + o + 3ull * rows);
-    const uint32_t c0 = ldg_stream(ec + o), c1 = ldg_stream(ec + o + rows);
-    const uint32_t c2 = ldg_stream(ec + o + 2ull * rows), c3 = ldg_stream(ec + o + 3ull * rows);
-    const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-    d = fma_t(a0, x0, d);
-    d = fma_t(a1, x1, d);
-    d = fma_t(a2, x2, d);
-    d = fma_t(a3, x3, d);
-    o += 4ull * rows;
+  for (; k + 8 <= w; k += 8) group(std::integral_constant<uint32_t, 8>{});
+  if (k + 4 <= w) {
+    group(std::integral_constant<uint32_t, 4>{});
+    k += 4;
   }
-  for (; k < w; k++, o += rows) d = fma_t(ldg_stream(ev + o), x[ldg_stream(ec + o)], d);
+  if (k + 2 <= w) {
+    group(std::integral_constant<uint32_t, 2>{});
+    k += 2;
+  }
+  if (k < w) group(std::integral_constant<uint32_t, 1>{});
   if constexpr (ATOMIC)
     unsafeAtomicAdd(y + r, d);
   else
