@@ -134,7 +134,7 @@ def precision_study(mfp, timer, flush, reps, cache):
             s_v = 4 if vt == "f32" else 2
             nb = b_alg(rows, rows, nnz, s_v, 4)
             best = None
-            for fmt in (("csr_vector", "csr_adaptive", "ell") if vt == "f32" else ("csr_vector", "ell")):
+            for fmt in ("csr_vector", "csr_adaptive", "ell"):
                 try:
                     p = mfp.Plan(M, fmt)
                 except mfp.MfpBudgetError:

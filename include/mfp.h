@@ -126,7 +126,9 @@ int mfp_matrix_destroy(mfp_matrix m);
 
 /* Copy of a matrix with its values converted to dtype (from f32/f64 values;
  * 16-bit types are rounded from the FP32 value, nearest-even).  No reference
- * counterpart: the value-precision study. */
+ * counterpart: the value-precision study.  Matrices with 16-bit values plan
+ * as CSR-vector, ELL, or CSR-Adaptive (rows binned by length, long rows in
+ * chunks); all accumulate in FP32 with Dot2 compensation. */
 int mfp_matrix_convert(mfp_matrix src, int dtype, mfp_matrix *out);
 
 /* FP64 oracle SpMV on the GPU: y64 = A x with the stored values and x

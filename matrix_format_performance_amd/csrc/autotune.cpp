@@ -63,7 +63,7 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
   auto want = [&](int f) { return (format_mask >> f) & 1u; };
   for (int f = 0; f < MFP_FORMAT_COUNT; f++) {
     if (!want(f)) continue;
-    if (lp && f != MFP_CSR_VECTOR && f != MFP_ELL) continue;
+    if (lp && f != MFP_CSR_VECTOR && f != MFP_ELL && f != MFP_CSR_ADAPTIVE) continue;
     mfp_tuning t;
     st = mfp_tuning_default(m, f, &t);
     if (st != MFP_OK) return st;
@@ -78,6 +78,10 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
       }
       break;
     case MFP_CSR_ADAPTIVE: {
+      if (lp) { // the binned 16-bit form has no block size
+        push(c, f, tagged);
+        break;
+      }
       const uint32_t base = t.nnz_per_wg;
       for (uint32_t w : {base / 2, base, base * 2}) {
         t.nnz_per_wg = w;

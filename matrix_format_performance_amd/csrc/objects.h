@@ -30,10 +30,13 @@ struct mfp_plan_s {
   uint64_t device_bytes = 0;
   // 16-bit-value ELL: columns as u16 offsets from base[row] when off16
   bool off16 = false;
+  bool lp_block = false; // 16-bit offsets from a base per 64-row block
   void *lp_cols = nullptr;
   uint32_t *lp_base = nullptr;
   // column-sorted SCOO (MFP_TUNE_SCOO_COLSORT)
   bool colsort = false;
   mfp::scoo_cs_layout cs;
+  // binned CSR over 16-bit values
+  mfp::lp_binned_layout lpb;
 };
 

@@ -82,12 +82,14 @@ using namespace mfp;
 namespace mfp {
 int launch_csr_vector_lp(int dt, uint32_t lanes, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
                          const void *a, const void *x, void *y, hipStream_t s);
-int launch_ell_lp(int dt, bool off16, uint32_t rows, uint32_t w, const void *cols, const uint32_t *base,
+int launch_ell_lp(int dt, bool off16, bool block, uint32_t rows, uint32_t w, const void *cols, const uint32_t *base,
                   const void *ev, const void *x, void *y, hipStream_t s);
+int block_base_build(uint32_t rows, const uint32_t *rp, const uint32_t *ci, uint32_t *base, uint32_t *span,
+                     hipStream_t s);
 int row_span_max(uint32_t rows, const uint32_t *rp, const uint32_t *ci, uint32_t *out, hipStream_t s);
 uint32_t ell_lp_stride(uint32_t rows);
 int ell_lp_build(uint32_t rows, const uint32_t *rp, const uint32_t *ci, const void *a, uint32_t w,
-                 bool off16, void *ev, void *cols, uint32_t *base, hipStream_t s);
+                 bool off16, bool block, void *ev, void *cols, uint32_t *base, hipStream_t s);
 int to_lp_values(const void *in, int din, void *out, int dout, uint64_t n, hipStream_t s);
 } // namespace mfp
 
@@ -145,6 +147,7 @@ static int plan_free_arrays(mfp_plan p)
   dfree(p->lp_cols);
   dfree(p->lp_base);
   scoo_cs_free(&p->cs);
+  lp_binned_free(&p->lpb);
   free_adaptive_layout(&p->ad);
   return MFP_OK;
 }
@@ -172,8 +175,13 @@ static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipS
     }
     return MFP_OK;
   }
+  if (p->format == MFP_CSR_ADAPTIVE) {
+    MFP_CHECK(lp_binned_build(m->rows, m->row_ptr, &p->lpb, s));
+    p->device_bytes = p->lpb.device_bytes;
+    return MFP_OK;
+  }
   if (p->format != MFP_ELL) {
-    set_error("16-bit values are available in the CSR-vector and ELL formats");
+    set_error("16-bit values are available in the CSR-vector, CSR-Adaptive (binned) and ELL formats");
     return MFP_ERR_ARG;
   }
   mfp_row_stats rs;
@@ -181,21 +189,41 @@ static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipS
   const uint32_t w = rs.max_elements_in_rows;
   uint32_t span = 0;
   MFP_CHECK(row_span_max(m->rows, m->row_ptr, m->columns, &span, s));
-  p->off16 = span < 65536u && !(t.flags & MFP_TUNE_NO_OFFSET16);
   const uint32_t stride = ell_lp_stride(m->rows); // rows rounded up to a multiple of 64
+  // offsets from the first column of each 64-row block when every block's
+  // columns span less than 65536 (one base per block), else per row
+  uint32_t *bbase = nullptr;
+  uint32_t bspan = 0xFFFFFFFFu;
+  if (!(t.flags & MFP_TUNE_NO_OFFSET16) && m->rows) {
+    MFP_CHECK(dmalloc((void **)&bbase, (uint64_t)stride / 64 * 4));
+    const int bst = block_base_build(m->rows, m->row_ptr, m->columns, bbase, &bspan, s);
+    if (bst != MFP_OK) {
+      dfree(bbase);
+      return bst;
+    }
+  }
+  p->lp_block = bspan < 65536u;
+  if (!p->lp_block) {
+    dfree(bbase);
+    bbase = nullptr;
+  }
+  p->off16 = p->lp_block || (span < 65536u && !(t.flags & MFP_TUNE_NO_OFFSET16));
   // with 16-bit offsets the slots are packed 4 per 16-byte word (w rounded up to 4)
   const uint64_t slots = (uint64_t)stride * (p->off16 ? (w + 3) / 4 * 4 : w);
-  const uint64_t need = slots * (p->off16 ? 4 : 6) + (p->off16 ? (uint64_t)stride * 4 : 0);
+  const uint64_t base_bytes = p->lp_block ? (uint64_t)stride / 64 * 4 : (p->off16 ? (uint64_t)stride * 4 : 0);
+  const uint64_t need = slots * (p->off16 ? 4 : 6) + base_bytes;
   if (need > budget_of(t)) {
+    dfree(bbase);
     set_error("ELL part needs " + std::to_string(need) + " bytes, above the budget");
     return MFP_ERR_BUDGET;
   }
+  p->lp_base = bbase; // the plan owns it from here
   p->ell_w = w;
   // with 16-bit offsets, value and offset share one u32 slot in lp_cols
   if (!p->off16) MFP_CHECK(dmalloc(&p->ell_val, slots * 2));
   MFP_CHECK(dmalloc(&p->lp_cols, slots * 4));
-  if (p->off16) MFP_CHECK(dmalloc((void **)&p->lp_base, (uint64_t)stride * 4));
-  MFP_CHECK(ell_lp_build(m->rows, m->row_ptr, m->columns, m->values, w, p->off16, p->ell_val,
+  if (p->off16 && !p->lp_block) MFP_CHECK(dmalloc((void **)&p->lp_base, (uint64_t)stride * 4));
+  MFP_CHECK(ell_lp_build(m->rows, m->row_ptr, m->columns, m->values, w, p->off16, p->lp_block, p->ell_val,
                          p->lp_cols, p->lp_base, s));
   MFP_HIP(hipStreamSynchronize(s));
   p->device_bytes = need;
@@ -538,7 +566,9 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
   if (is_lp(dt)) {
     if (p->format == MFP_CSR_VECTOR)
       return launch_csr_vector_lp(dt, p->t.lanes, m->rows, m->row_ptr, m->columns, m->values, x, y, s);
-    return launch_ell_lp(dt, p->off16, m->rows, p->ell_w, p->lp_cols, p->lp_base, p->ell_val, x, y, s);
+    if (p->format == MFP_CSR_ADAPTIVE)
+      return launch_lp_binned(dt, p->lpb, m->row_ptr, m->columns, m->values, x, y, s);
+    return launch_ell_lp(dt, p->off16, p->lp_block, m->rows, p->ell_w, p->lp_cols, p->lp_base, p->ell_val, x, y, s);
   }
   switch (p->format) {
   case MFP_CSR_SCALAR:

@@ -20,6 +20,9 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <vector>
+
+#include <algorithm>
 #include <cmath>
 
 namespace mfp {
@@ -179,17 +182,19 @@ __global__ void k_to_lp(const void *in, int din, uint16_t *__restrict__ out, int
 }
 
 // ---- low-precision CSR-vector (G lanes per row, Dot2 per lane) --------------------------
-template <int DT, int G>
-__global__ __launch_bounds__(256) void k_csr_vector_lp(uint32_t rows, const uint32_t *__restrict__ rp,
-                                                       const uint32_t *__restrict__ ci,
-                                                       const uint16_t *__restrict__ a,
-                                                       const float *__restrict__ x, float *__restrict__ y)
+// G lanes per row, Dot2 per lane, compensated group reduction.  Block blk's
+// group g takes row g (or list[g] with a row list); n rows in all.
+template <int DT, int G, bool LIST>
+__device__ __forceinline__ void lp_rows(uint32_t blk, uint32_t n, const uint32_t *__restrict__ rp,
+                                        const uint32_t *__restrict__ ci, const uint16_t *__restrict__ a,
+                                        const float *__restrict__ x, float *__restrict__ y,
+                                        const uint32_t *__restrict__ list)
 {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t row = (blk * 256u + threadIdx.x) / G;
+  const uint32_t g = (blk * 256u + threadIdx.x) / G;
   const uint32_t lane = threadIdx.x % G;
+  const uint32_t row = LIST ? (g < n ? list[g] : 0xFFFFFFFFu) : g;
   comp acc{0.f, 0.f};
-  if (row < rows) {
+  if (g < n) {
     const uint32_t b = rp[row], e = rp[row + 1];
     uint32_t i = b + lane;
     for (; i + 3u * G < e; i += 4u * G) {
@@ -206,7 +211,76 @@ __global__ __launch_bounds__(256) void k_csr_vector_lp(uint32_t rows, const uint
     for (; i < e; i += G) dot2_add(acc, lp_val<DT>(ldg_stream(a + i)), x[ldg_stream(ci + i)]);
   }
   acc = comp_group_reduce<G>(acc);
-  if (lane == 0 && row < rows) y[row] = acc.s + acc.c;
+  if (lane == 0 && g < n) y[row] = acc.s + acc.c;
+}
+
+template <int DT, int G>
+__global__ __launch_bounds__(256) void k_csr_vector_lp(uint32_t rows, const uint32_t *__restrict__ rp,
+                                                       const uint32_t *__restrict__ ci,
+                                                       const uint16_t *__restrict__ a,
+                                                       const float *__restrict__ x, float *__restrict__ y)
+{
+  lp_rows<DT, G, false>(xcd_block(blockIdx.x, gridDim.x), rows, rp, ci, a, x, y, nullptr);
+}
+
+// Chunks of long rows: one wave per chunk {row, first entry, end entry,
+// slot}; the wave's compensated sum goes to partial[slot].
+template <int DT>
+__device__ __forceinline__ void lp_chunks(uint32_t blk, uint32_t n, const uint4 *__restrict__ chunks,
+                                          const uint32_t *__restrict__ ci, const uint16_t *__restrict__ a,
+                                          const float *__restrict__ x, float2 *__restrict__ partial)
+{
+  const uint32_t c = blk * 4u + threadIdx.x / 64u;
+  const uint32_t lane = threadIdx.x & 63u;
+  comp acc{0.f, 0.f};
+  uint4 ch = make_uint4(0, 0, 0, 0);
+  if (c < n) {
+    ch = chunks[c];
+    uint32_t i = ch.y + lane;
+    for (; i + 3u * 64u < ch.z; i += 4u * 64u) {
+      const uint16_t a0 = ldg_stream(a + i), a1 = ldg_stream(a + i + 64), a2 = ldg_stream(a + i + 128),
+                     a3 = ldg_stream(a + i + 192);
+      const uint32_t c0 = ldg_stream(ci + i), c1 = ldg_stream(ci + i + 64), c2 = ldg_stream(ci + i + 128),
+                     c3 = ldg_stream(ci + i + 192);
+      const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+      dot2_add(acc, lp_val<DT>(a0), x0);
+      dot2_add(acc, lp_val<DT>(a1), x1);
+      dot2_add(acc, lp_val<DT>(a2), x2);
+      dot2_add(acc, lp_val<DT>(a3), x3);
+    }
+    for (; i < ch.z; i += 64u) dot2_add(acc, lp_val<DT>(ldg_stream(a + i)), x[ldg_stream(ci + i)]);
+  }
+  acc = comp_group_reduce<64>(acc);
+  if (lane == 0 && c < n) partial[ch.w] = make_float2(acc.s, acc.c);
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void k_lp_chunks(uint32_t n, const uint4 *__restrict__ chunks,
+                                                   const uint32_t *__restrict__ ci, const uint16_t *__restrict__ a,
+                                                   const float *__restrict__ x, float2 *__restrict__ partial)
+{
+  lp_chunks<DT>(xcd_block(blockIdx.x, gridDim.x), n, chunks, ci, a, x, partial);
+}
+
+template <int DT, int G>
+__global__ __launch_bounds__(256) void k_lp_rows(uint32_t n, const uint32_t *__restrict__ list,
+                                                 const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                                                 const uint16_t *__restrict__ a, const float *__restrict__ x,
+                                                 float *__restrict__ y)
+{
+  lp_rows<DT, G, true>(xcd_block(blockIdx.x, gridDim.x), n, rp, ci, a, x, y, list);
+}
+
+// y[row] of each long row {row, first slot, slots}: its chunks merged in order
+__global__ void k_lp_long_finish(uint32_t n, const uint4 *__restrict__ longs, const float2 *__restrict__ partial,
+                                 float *__restrict__ y)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint4 l = longs[i];
+  comp acc{partial[l.y].x, partial[l.y].y};
+  for (uint32_t k = 1; k < l.z; k++) acc = comp_merge(acc, comp{partial[l.y + k].x, partial[l.y + k].y});
+  y[l.x] = acc.s + acc.c;
 }
 
 // ---- low-precision ELL (Dot2, optional 16-bit column offsets) -------------------------------
@@ -268,7 +342,7 @@ struct ell_lp_row16 {
   }
 };
 
-template <int DT, bool OFF16>
+template <int DT, bool OFF16, bool BLOCK = false>
 __global__ __launch_bounds__(256) void k_ell_lp(uint32_t rows, uint32_t stride, uint32_t w,
                                                 const void *__restrict__ cols, const uint32_t *__restrict__ base,
                                                 const uint16_t *__restrict__ ev, const float *__restrict__ x,
@@ -281,7 +355,8 @@ __global__ __launch_bounds__(256) void k_ell_lp(uint32_t rows, uint32_t stride, 
   // groups of slots with no guarded loads, so each group's loads are in flight together
   if constexpr (OFF16) {
     const uint32_t w4 = (w + 3) / 4;
-    const ell_lp_row16<DT> row{(const u32x4v *)cols, x, r, w4, base[r]};
+    // offsets from the row's first column, or from its 64-row block's
+    const ell_lp_row16<DT> row{(const u32x4v *)cols, x, r, w4, BLOCK ? base[r >> 6] : base[r]};
     uint32_t j = 0;
     for (; j + 3 <= w4; j += 3) row.template group<3>(j, acc);
     if (j + 2 <= w4) {
@@ -324,16 +399,57 @@ __global__ void k_row_span(uint32_t rows, const uint32_t *__restrict__ rp, const
   atomicMax(mx, m);
 }
 
+// Per 64-row block: the smallest first column of its rows (0 for a block of
+// empty rows) into base[], and the block's column span into *mx.
+__global__ void k_block_base(uint32_t rows, uint32_t nblocks, const uint32_t *__restrict__ rp,
+                             const uint32_t *__restrict__ ci, uint32_t *__restrict__ base, uint32_t *mx)
+{
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  uint32_t lo = 0xFFFFFFFFu, hi = 0;
+  for (uint32_t r = b * 64u; r < min(rows, b * 64u + 64u); r++) {
+    const uint32_t s = rp[r], e = rp[r + 1];
+    if (e > s) {
+      lo = min(lo, ci[s]);
+      hi = max(hi, ci[e - 1]);
+    }
+  }
+  base[b] = lo == 0xFFFFFFFFu ? 0u : lo;
+  if (lo != 0xFFFFFFFFu) atomicMax(mx, hi - lo);
+}
+
+int block_base_build(uint32_t rows, const uint32_t *rp, const uint32_t *ci, uint32_t *base, uint32_t *span,
+                     hipStream_t s)
+{
+  uint32_t *d = nullptr;
+  MFP_CHECK(dmalloc((void **)&d, 4));
+  int st = fill_u32(d, 0, 1, s);
+  const uint32_t nb = (uint32_t)(((uint64_t)rows + 63) / 64);
+  if (st == MFP_OK && nb) {
+    k_block_base<<<ceil_div_u32(nb, 256), 256, 0, s>>>(rows, nb, rp, ci, base, d);
+    if (hipGetLastError() != hipSuccess) st = MFP_ERR_HIP;
+  }
+  if (st == MFP_OK && (hipMemcpyAsync(span, d, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                       hipStreamSynchronize(s) != hipSuccess))
+    st = MFP_ERR_HIP;
+  dfree(d);
+  return st;
+}
+
 // ELL build for 16-bit values: column-major, padding (0, offset 0 / col 0).
 __global__ void k_ell_lp_build(uint32_t rows, uint32_t stride, const uint32_t *__restrict__ rp,
                                const uint32_t *__restrict__ ci, const uint16_t *__restrict__ a, uint32_t w,
-                               bool off16, uint16_t *__restrict__ ev, void *cols, uint32_t *__restrict__ base)
+                               bool off16, bool block, uint16_t *__restrict__ ev, void *cols,
+                               uint32_t *__restrict__ base)
 {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= stride) return;
   const uint32_t s = r < rows ? rp[r] : 0u, len = r < rows ? rp[r + 1] - s : 0u;
-  const uint32_t b0 = len ? ci[s] : 0u;
-  if (off16) base[r] = b0;
+  uint32_t b0 = len ? ci[s] : 0u;
+  if (off16 && block)
+    b0 = base[r >> 6];
+  else if (off16)
+    base[r] = b0;
   const uint32_t kend = off16 ? (w + 3) / 4 * 4 : w; // packed words also hold the padding slots
   for (uint32_t k = 0; k < kend; k++) {
     const bool in = k < len;
@@ -397,7 +513,7 @@ int launch_csr_vector_lp(int dt, uint32_t lanes, uint32_t rows, const uint32_t *
 
 uint32_t ell_lp_stride(uint32_t rows) { return (uint32_t)(((uint64_t)rows + 63) / 64 * 64); }
 
-int launch_ell_lp(int dt, bool off16, uint32_t rows, uint32_t w, const void *cols, const uint32_t *base,
+int launch_ell_lp(int dt, bool off16, bool block, uint32_t rows, uint32_t w, const void *cols, const uint32_t *base,
                   const void *ev, const void *x, void *y, hipStream_t s)
 {
   if (rows == 0) return MFP_OK;
@@ -407,13 +523,117 @@ int launch_ell_lp(int dt, bool off16, uint32_t rows, uint32_t w, const void *col
   const float *xv = (const float *)x;
   float *yv = (float *)y;
   if (dt == MFP_BF16) {
-    if (off16) k_ell_lp<MFP_BF16, true><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
+    if (off16 && block) k_ell_lp<MFP_BF16, true, true><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
+    else if (off16) k_ell_lp<MFP_BF16, true><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
     else k_ell_lp<MFP_BF16, false><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
   } else {
-    if (off16) k_ell_lp<MFP_F16, true><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
+    if (off16 && block) k_ell_lp<MFP_F16, true, true><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
+    else if (off16) k_ell_lp<MFP_F16, true><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
     else k_ell_lp<MFP_F16, false><<<grid, 256, 0, s>>>(rows, stride, w, cols, base, e, xv, yv);
   }
   return launched_lp();
+}
+
+// Binned 16-bit CSR: rows grouped by length into lists for G = 2..64 lanes
+// (G = the row length rounded up to a power of two), rows longer than
+// LP_LONG split into chunks of LP_CHUNK entries, one wave each.
+constexpr uint32_t LP_LONG = 4096, LP_CHUNK = 1024;
+
+int lp_binned_build(uint32_t rows, const uint32_t *rp_dev, lp_binned_layout *out, hipStream_t s)
+{
+  *out = lp_binned_layout{};
+  std::vector<uint32_t> rp((size_t)rows + 1);
+  MFP_HIP(hipMemcpyAsync(rp.data(), rp_dev, rp.size() * 4, hipMemcpyDeviceToHost, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> bins[6];
+  std::vector<uint4> chunks, longs;
+  for (uint32_t r = 0; r < rows; r++) {
+    const uint32_t len = rp[r + 1] - rp[r];
+    if (len > LP_LONG) {
+      const uint32_t n = (len + LP_CHUNK - 1) / LP_CHUNK;
+      longs.push_back(make_uint4(r, (uint32_t)chunks.size(), n, 0));
+      for (uint32_t k = 0; k < n; k++)
+        chunks.push_back(make_uint4(r, rp[r] + k * LP_CHUNK, std::min(rp[r + 1], rp[r] + (k + 1) * LP_CHUNK),
+                                    (uint32_t)chunks.size()));
+      continue;
+    }
+    uint32_t b = 0;
+    while (b < 5 && (2u << b) < len) b++;
+    bins[b].push_back(r);
+  }
+  std::vector<uint32_t> list;
+  for (int b = 0; b < 6; b++) {
+    out->bin_off[b] = (uint32_t)list.size();
+    list.insert(list.end(), bins[b].begin(), bins[b].end());
+  }
+  out->bin_off[6] = (uint32_t)list.size();
+  out->nchunks = (uint32_t)chunks.size();
+  out->nlong = (uint32_t)longs.size();
+  int st = dmalloc((void **)&out->list, std::max<size_t>(list.size(), 1) * 4);
+  if (st == MFP_OK) st = dmalloc((void **)&out->chunks, std::max<size_t>(chunks.size(), 1) * sizeof(uint4));
+  if (st == MFP_OK) st = dmalloc((void **)&out->longs, std::max<size_t>(longs.size(), 1) * sizeof(uint4));
+  if (st == MFP_OK) st = dmalloc((void **)&out->partial, std::max<size_t>(chunks.size(), 1) * sizeof(float2));
+  if (st != MFP_OK) {
+    lp_binned_free(out);
+    return st;
+  }
+  if (!list.empty()) MFP_HIP(hipMemcpyAsync(out->list, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
+  if (!chunks.empty())
+    MFP_HIP(hipMemcpyAsync(out->chunks, chunks.data(), chunks.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+  if (!longs.empty())
+    MFP_HIP(hipMemcpyAsync(out->longs, longs.data(), longs.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  out->device_bytes = list.size() * 4 + (chunks.size() + longs.size()) * sizeof(uint4) + chunks.size() * 8;
+  return MFP_OK;
+}
+
+void lp_binned_free(lp_binned_layout *l)
+{
+  dfree(l->list);
+  dfree(l->chunks);
+  dfree(l->longs);
+  dfree(l->partial);
+  *l = lp_binned_layout{};
+}
+
+// one launch per bin (a single launch over all bins measured slower: its
+// registers are those of the widest variant), long-row chunks first
+template <int DT>
+static int lp_binned_t(const lp_binned_layout &l, const uint32_t *rp, const uint32_t *ci, const uint16_t *a,
+                       const float *x, float *y, hipStream_t s)
+{
+  if (l.nchunks) {
+    k_lp_chunks<DT><<<ceil_div_u32(l.nchunks, 4), 256, 0, s>>>(l.nchunks, l.chunks, ci, a, x, l.partial);
+    MFP_CHECK(launched_lp());
+  }
+  for (int b = 0; b < 6; b++) {
+    const uint32_t n = l.bin_off[b + 1] - l.bin_off[b];
+    if (!n) continue;
+    const uint32_t grid = ceil_div_u32((uint64_t)n * (2u << b), 256);
+    const uint32_t *lst = l.list + l.bin_off[b];
+    switch (b) {
+    case 0: k_lp_rows<DT, 2><<<grid, 256, 0, s>>>(n, lst, rp, ci, a, x, y); break;
+    case 1: k_lp_rows<DT, 4><<<grid, 256, 0, s>>>(n, lst, rp, ci, a, x, y); break;
+    case 2: k_lp_rows<DT, 8><<<grid, 256, 0, s>>>(n, lst, rp, ci, a, x, y); break;
+    case 3: k_lp_rows<DT, 16><<<grid, 256, 0, s>>>(n, lst, rp, ci, a, x, y); break;
+    case 4: k_lp_rows<DT, 32><<<grid, 256, 0, s>>>(n, lst, rp, ci, a, x, y); break;
+    default: k_lp_rows<DT, 64><<<grid, 256, 0, s>>>(n, lst, rp, ci, a, x, y); break;
+    }
+    MFP_CHECK(launched_lp());
+  }
+  if (l.nchunks) {
+    k_lp_long_finish<<<ceil_div_u32(l.nlong, 256), 256, 0, s>>>(l.nlong, l.longs, l.partial, y);
+    MFP_CHECK(launched_lp());
+  }
+  return MFP_OK;
+}
+
+int launch_lp_binned(int dt, const lp_binned_layout &l, const uint32_t *rp, const uint32_t *ci, const void *a,
+                     const void *x, void *y, hipStream_t s)
+{
+  if (dt == MFP_BF16)
+    return lp_binned_t<MFP_BF16>(l, rp, ci, (const uint16_t *)a, (const float *)x, (float *)y, s);
+  return lp_binned_t<MFP_F16>(l, rp, ci, (const uint16_t *)a, (const float *)x, (float *)y, s);
 }
 
 int row_span_max(uint32_t rows, const uint32_t *rp, const uint32_t *ci, uint32_t *out, hipStream_t s)
@@ -429,12 +649,12 @@ int row_span_max(uint32_t rows, const uint32_t *rp, const uint32_t *ci, uint32_t
 }
 
 int ell_lp_build(uint32_t rows, const uint32_t *rp, const uint32_t *ci, const void *a, uint32_t w,
-                 bool off16, void *ev, void *cols, uint32_t *base, hipStream_t s)
+                 bool off16, bool block, void *ev, void *cols, uint32_t *base, hipStream_t s)
 {
   if (rows == 0 || w == 0) return MFP_OK;
   const uint32_t stride = ell_lp_stride(rows);
   k_ell_lp_build<<<ceil_div_u32(stride, 256), 256, 0, s>>>(rows, stride, rp, ci, (const uint16_t *)a, w, off16,
-                                                           (uint16_t *)ev, cols, base);
+                                                           block, (uint16_t *)ev, cols, base);
   return launched_lp();
 }
 

@@ -64,8 +64,8 @@ def test_value_conversion_is_round_to_nearest_even(gpu, vt):
 
 @pytest.mark.parametrize("vt", ["bf16", "f16"])
 @pytest.mark.parametrize("fmt,kw", [("csr_vector", {"lanes": 4}), ("csr_vector", {"lanes": 32}),
-                                    ("ell", {}), ("ell", {"flags": 1})])
-@pytest.mark.parametrize("name", ["stencil3d_20", "uniform_band", "edge_rows", "stencil2d_64"])
+                                    ("ell", {}), ("ell", {"flags": 1}), ("csr_adaptive", {})])
+@pytest.mark.parametrize("name", ["stencil3d_20", "uniform_band", "edge_rows", "stencil2d_64", "powerlaw_20k"])
 def test_lp_spmv_compensated_accuracy(gpu, name, vt, fmt, kw):
     rp, ci, va, cols = case(name, "float32")
     announce(f"lp spmv {vt} {fmt} {kw} case={name}")
@@ -78,6 +78,8 @@ def test_lp_spmv_compensated_accuracy(gpu, name, vt, fmt, kw):
     info = plan.info()
     if fmt == "ell":
         assert info["column_bytes"] == (4 if kw.get("flags") else 2)
+    if fmt == "ell" and name == "powerlaw_20k":
+        pytest.skip("a 12,000-wide ELL is legal but slow to check here")
     plan.spmv(xd, yd)
     gpu.synchronize()
     y = yd.to_host()
@@ -97,7 +99,7 @@ def test_lp_compensation_survives_cancellation(gpu):
     x = np.ones(3, np.float32)
     A = gpu.DeviceMatrix.from_csr(rp, ci, va, 3).convert("bf16")
     xd = gpu.DeviceArray.from_host(x)
-    for fmt in ("csr_vector", "ell"):
+    for fmt in ("csr_vector", "ell", "csr_adaptive"):
         yd = gpu.DeviceArray(rows, np.float32)
         gpu.Plan(A, fmt, **({"lanes": 2} if fmt == "csr_vector" else {})).spmv(xd, yd)
         gpu.synchronize()
