@@ -189,6 +189,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 / FP16 study")
+    ap.add_argument("--values", default=None, choices=["bf16", "f16"],
+                    help="store the matrix values in 16 bits (FP32 vectors and accumulation)")
     ap.add_argument("--bandwidth", type=float, default=1.0,
                     help="power-law only: column window as a fraction of cols (study knob)")
     args = ap.parse_args()
@@ -215,11 +217,16 @@ def main():
     rows, nnz = rp.size - 1, ci.size
     cols = rows
     x = mfp.generate_vector(1, cols, dtype)
-    nbytes = b_alg(rows, cols, nnz, s, s)
+    nbytes = b_alg(rows, cols, nnz, 2 if args.values else s, s)
     log(f"[bench] workload {args.workload}: rows={rows} nnz={nnz} B_alg={nbytes / 1e6:.1f} MB "
         f"(generated in {time.time() - t0:.1f}s)")
 
     A = mfp.DeviceMatrix.from_csr(rp, ci, va, cols)
+    if args.values:
+        if args.dtype != "f32":
+            sys.exit("--values needs --dtype f32 (16-bit values use FP32 vectors)")
+        A32, A = A, A.convert(args.values)
+        A32.destroy()
     xd = mfp.DeviceArray.from_host(x)
     yd = mfp.DeviceArray(rows, dtype)
     timer = mfp.Timer()
@@ -283,8 +290,14 @@ def main():
     mfp.synchronize()
     import oracle
     y = yd.to_host()
-    y_ref = oracle.spmv_seq(rp, ci, va, x)
-    _, absdot = oracle.spmv_f64(rp, ci, va, x)
+    va_stored = va
+    if args.values == "bf16":
+        from oracle.selector import bf16_round
+        va_stored = bf16_round(va)
+    elif args.values == "f16":
+        va_stored = va.astype(np.float16).astype(np.float32)
+    y_ref = oracle.spmv_seq(rp, ci, va_stored, x)
+    _, absdot = oracle.spmv_f64(rp, ci, va_stored, x)
     err = oracle.tolerance_check(y, y_ref, absdot, 1.0)
 
     for _ in range(args.warmup):
@@ -298,7 +311,8 @@ def main():
     value = nbytes / (mean_ms * 1e-3) / 1e9
 
     traffic = None
-    tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.dtype}_{best}.json")
+    vtag = args.values or args.dtype
+    tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{vtag}_{best}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -320,6 +334,7 @@ def main():
             "workload": args.workload,
             "description": WORKLOADS[args.workload],
             "rows": rows, "cols": cols, "nnz": nnz,
+            "values": args.values or args.dtype,
             "format": best,
             "tuning": tuning_summary(best, best_c["tuning"]),
             "selection": f"exhaustive autotune, {len(cands)} candidates, {t_sel:.1f} s",
@@ -350,6 +365,9 @@ def main():
     mfp.synchronize()
     t_h = time.perf_counter()
     A2 = mfp.DeviceMatrix.from_csr(rp, ci, va, cols)
+    if args.values:
+        A2f, A2 = A2, A2.convert(args.values)
+        A2f.destroy()
     x2 = mfp.DeviceArray.from_host(x)
     y2 = mfp.DeviceArray(rows, dtype)
     p2 = mfp.Plan(A2, best, **plan_kw)
@@ -363,7 +381,7 @@ def main():
         o.destroy() if hasattr(o, "destroy") else o.free()
     plan.destroy()
 
-    if args.dtype == "f32" and not args.no_precision:
+    if args.dtype == "f32" and not args.values and not args.no_precision:
         cache = {args.workload: (rp, ci, va)} if args.bandwidth >= 1.0 else {}
         A.destroy()
         out["precision"] = precision_study(mfp, timer, flush, args.select_reps, cache)
