@@ -41,7 +41,96 @@ WORKLOADS = {
                    "uniform distinct columns, values and x U[-1,1)",
     "stencil3d_128": "3D 27-point stencil 128^3 (2,097,152 rows, 55,742,968 nnz), values N(0,1)",
     "stencil2d_256": "2D 5-point Laplacian 256^2 (65,536 rows, 326,656 nnz)",
+    "batch512": "512 matrices, rows log-uniform in [1k, 200k], 2D/3D stencils, uniform 4-64 nnz/row and "
+                "power-law alpha in [1.5, 3] (max 10,000 per row), FP32, k = 1, 4, 16 right-hand sides in one "
+                "persistent launch replayed as a HIP graph",
 }
+
+
+def batch_members(mfp, count=512, seed=0):
+    """The batch workload's matrices (host CSR, FP32)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        rows = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+        fam = ("stencil2d", "stencil3d", "uniform", "powerlaw")[i % 4]
+        if fam == "stencil2d":
+            spec = mfp.GenSpec("stencil2d", values="uniform", n=max(2, int(np.sqrt(rows))), seed=i)
+        elif fam == "stencil3d":
+            spec = mfp.GenSpec("stencil3d", values="normal", n=max(2, int(round(rows ** (1 / 3)))), seed=i)
+        elif fam == "uniform":
+            spec = mfp.GenSpec("uniform", values="uniform", rows=rows, nnz_lo=4, nnz_hi=64, seed=i)
+        else:
+            spec = mfp.GenSpec("powerlaw", values="uniform", rows=rows, nnz_lo=1, nnz_hi=min(rows // 2, 10000),
+                               alpha=float(rng.uniform(1.5, 3.0)), seed=i)
+        out.append(mfp.generate(spec, np.float32))
+    return out
+
+
+def run_batch(mfp, args, timer):
+    """One step = one replay of the captured persistent launch over all 512
+    matrices (k right-hand sides); value = algorithmic bytes / mean step."""
+    t0 = time.time()
+    members = batch_members(mfp)
+    dms = [mfp.DeviceMatrix.from_csr(rp, ci, va) for rp, ci, va in members]
+    # lanes per row for the k > 1 launch: the mean row length rounded up to a power of two
+    tot_rows = sum(m[0].size - 1 for m in members)
+    tot_nnz = sum(m[1].size for m in members)
+    lanes = args.batch_lanes or int(min(64, max(2, 1 << max(0, int(np.ceil(np.log2(max(1.0, tot_nnz / tot_rows))))))))
+    batch = mfp.Batch(dms, lanes=lanes)
+    for d in dms:
+        d.destroy()
+    rows, cols, nnz = batch.rows, batch.cols, batch.nnz
+    log(f"[bench] batch of {batch.count}: rows={rows} nnz={nnz} (built in {time.time() - t0:.1f}s)")
+    del members
+    res = {}
+    for k in (1, 4, 16):
+        X = mfp.DeviceArray.from_host(mfp.generate_vector(2, cols * k, np.float32))
+        Y = mfp.DeviceArray(rows * k, np.float32)
+        g = batch.graph(k, X, Y)
+        for _ in range(args.warmup):
+            mfp.flush_cache(1 << 30)
+            g.launch()
+        mfp.synchronize()
+        ms = []
+        for _ in range(args.steps):
+            mfp.flush_cache(1 << 30)
+            timer.start()
+            g.launch()
+            timer.stop()
+            ms.append(timer.ms())
+        mean_ms = statistics.fmean(ms)
+        nb = nnz * 8 + (rows + 1) * 4 + k * (cols + rows) * 4
+        res[k] = {"ms": round(mean_ms, 5), "GB_per_s": round(nb / (mean_ms * 1e-3) / 1e9, 1),
+                  "gflops": round(2 * nnz * k / (mean_ms * 1e-3) / 1e9, 1), "algorithmic_bytes": nb}
+        if k == 1:
+            # against the FP64 oracle on the GPU over the assembled matrix
+            m = mfp.engine.C.c_void_p()
+            mfp._lib.call("mfp_batch_matrix", batch.handle, mfp.engine.C.byref(m))
+            A = mfp.DeviceMatrix(m.value)
+            y64, ad = A.oracle_f64(X)
+            res[k]["max_row_err_vs_fp64"] = mfp.residual(Y, y64, ad)["max_row_err"]
+            A.handle = None  # owned by the batch
+        log(f"[bench] batch k={k}: {mean_ms * 1e3:.1f} us {res[k]['GB_per_s']} GB/s {res[k]['gflops']} GFLOP/s")
+        g.destroy()
+        X.free()
+        Y.free()
+    value = res[1]["GB_per_s"]
+    out = {
+        "metric": METRIC, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res[1]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch512", "description": WORKLOADS["batch512"], "matrices": batch.count,
+                   "spmm_lanes_per_row": lanes,
+                   "rows": rows, "cols": cols, "nnz": nnz, "launch": "one persistent launch, HIP graph replay",
+                   "cache": "Infinity Cache evicted before every step"},
+        "pct_hbm_roofline": round(100 * value / HBM_PEAK_GBS, 2),
+        "batch": res,
+        "roofline": {"bound": "hbm", "achieved": value, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(value / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    batch.destroy()
+    print(json.dumps(out), flush=True)
 
 
 def tuning_summary(fmt, t):
@@ -189,6 +278,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 / FP16 study")
+    ap.add_argument("--batch-lanes", type=int, default=0, help="batch512: lanes per row for k > 1 (0 = auto)")
     ap.add_argument("--values", default=None, choices=["bf16", "f16"],
                     help="store the matrix values in 16 bits (FP32 vectors and accumulation)")
     ap.add_argument("--bandwidth", type=float, default=1.0,
@@ -208,6 +298,9 @@ def main():
     info = mfp.device_info()
     log(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}")
     t0 = time.time()
+    if args.workload == "batch512":
+        run_batch(mfp, args, mfp.Timer())
+        return
     if args.workload == "powerlaw_1m" and args.bandwidth < 1.0:
         rp, ci, va = mfp.generate(mfp.GenSpec("powerlaw", values="uniform", rows=1 << 20, nnz_lo=1,
                                               nnz_hi=50000, alpha=2.0, seed=0,

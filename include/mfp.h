@@ -188,7 +188,8 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
 /* y = A x on stream s (no host sync; graph-capturable). */
 int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream s);
 /* Y = A X for k right-hand sides (k in {1,2,4,8,16}); X is cols x k and Y
- * rows x k, row-major.  k > 1 runs on CSR-vector plans (f32 / f64); each
+ * rows x k, row-major; X is 16-byte aligned when k values make a multiple
+ * of 16 bytes.  k > 1 runs on CSR-vector plans (f32 / f64); each
  * column is summed in the CSR-vector order.  No reference counterpart (the
  * reference multiplies one vector). */
 int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream s);

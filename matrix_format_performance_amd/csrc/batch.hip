@@ -20,6 +20,29 @@
 
 namespace mfp {
 
+// A row of X (KV values, aligned to KV * sizeof(T)) with 16-byte loads where
+// the row is a multiple of 16 bytes.
+template <typename T, int KV>
+__device__ __forceinline__ void load_row(const T *__restrict__ p, T (&out)[KV])
+{
+  if constexpr ((KV * sizeof(T)) % 16 == 0) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    constexpr int N = KV * sizeof(T) / 16;
+    constexpr int PER = 16 / sizeof(T);
+#pragma unroll
+    for (int v = 0; v < N; v++) {
+      const u4 w = *reinterpret_cast<const u4 *>(p + v * PER);
+      T tmp[PER];
+      __builtin_memcpy(tmp, &w, 16);
+#pragma unroll
+      for (int j = 0; j < PER; j++) out[v * PER + j] = tmp[j];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < KV; j++) out[j] = p[j];
+  }
+}
+
 template <typename T, int G, int KV, bool PERSISTENT>
 __global__ __launch_bounds__(256) void k_spmm_vector(uint32_t rows, const uint32_t *__restrict__ rp,
                                                      const uint32_t *__restrict__ ci,
@@ -36,12 +59,28 @@ __global__ __launch_bounds__(256) void k_spmm_vector(uint32_t rows, const uint32
     for (int j = 0; j < KV; j++) acc[j] = 0;
     if (row < rows) {
       const uint32_t b = rp[row], e = rp[row + 1];
-      for (uint32_t i = b + lane; i < e; i += G) {
-        const T av = ldg_stream(a + i);
-        const T *xr = X + (uint64_t)ldg_stream(ci + i) * KV;
-        T xv[KV];
+      uint32_t i = b + lane;
+      // four entries per lane in flight (matrix and X rows), then fma in entry order
+      for (; i + 3u * G < e; i += 4u * G) {
+        T av[4];
+        const T *xr[4];
 #pragma unroll
-        for (int j = 0; j < KV; j++) xv[j] = xr[j];
+        for (int q = 0; q < 4; q++) {
+          av[q] = ldg_stream(a + i + q * G);
+          xr[q] = X + (uint64_t)ldg_stream(ci + i + q * G) * KV;
+        }
+        T xv[4][KV];
+#pragma unroll
+        for (int q = 0; q < 4; q++) load_row<T, KV>(xr[q], xv[q]);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int j = 0; j < KV; j++) acc[j] = fma_t(av[q], xv[q][j], acc[j]);
+      }
+      for (; i < e; i += G) {
+        const T av = ldg_stream(a + i);
+        T xv[KV];
+        load_row<T, KV>(X + (uint64_t)ldg_stream(ci + i) * KV, xv);
 #pragma unroll
         for (int j = 0; j < KV; j++) acc[j] = fma_t(av, xv[j], acc[j]);
       }
@@ -91,6 +130,10 @@ int launch_spmm_vector(int dtype, uint32_t k, uint32_t lanes, uint32_t rows, con
                        hipStream_t s)
 {
   if (rows == 0) return MFP_OK;
+  if ((k * dtype_size(dtype)) % 16 == 0 && ((uintptr_t)X % 16) != 0) {
+    set_error("X must be 16-byte aligned when its rows are a multiple of 16 bytes");
+    return MFP_ERR_ARG;
+  }
   const uint32_t nblk = ceil_div_u32((uint64_t)rows * lanes, 256);
   uint32_t grid = nblk;
   if (persistent) grid = std::min<uint32_t>(nblk, (uint32_t)std::max(1, props().cu_count) * 8u);
