@@ -3,7 +3,8 @@
 // Swept parameters: lanes per row (CSR-vector, 2..64), CSR-Adaptive block
 // size, ELL width (longest row), the ELL/COO split of the hybrids (row-length
 // quantiles), the SCOO slice size (lane count filling a 32 KB LDS
-// accumulator) and column-sorted SCOO slices of 4096..32768 rows.  Every candidate is built as a plan and timed with an event
+// accumulator) and column-sorted SCOO slices of 4096..16384 rows (whole or
+// in 2 or 4 pieces per slice).  Every candidate is built as a plan and timed with an event
 // pair (median of reps, each after an Infinity-Cache flush when asked).
 #include "../../include/mfp.h"
 
@@ -101,16 +102,17 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
           continue;
         push(c, f, t);
       }
-      // column-sorted slices, pieces chosen by the plan
-      const uint64_t vsize = v.dtype == MFP_F64 ? 8 : 4;
-      for (uint32_t R : {4096u, 8192u, 16384u, 32768u}) {
-        if (R * vsize > 163840) break;
+      // column-sorted slices (8-byte LDS accumulators): pieces chosen by
+      // the plan, or 2 and 4 equal pieces per slice
+      for (uint32_t R : {4096u, 8192u, 16384u}) {
         if (v.rows && R > 4096 && R / 2 >= v.rows) break;
-        mfp_tuning tc = def;
-        tc.flags = MFP_TUNE_SCOO_COLSORT;
-        tc.scoo_slice_size = R;
-        tc.scoo_pieces = 0;
-        push(c, f, tc);
+        for (uint32_t P : {0u, 2u, 4u}) {
+          mfp_tuning tc = def;
+          tc.flags = MFP_TUNE_SCOO_COLSORT;
+          tc.scoo_slice_size = R;
+          tc.scoo_pieces = P;
+          push(c, f, tc);
+        }
       }
       break;
     }

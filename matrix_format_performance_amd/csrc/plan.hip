@@ -401,16 +401,15 @@ int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *t)
 }
 
 // Column-sorted SCOO: slice_size rows per slice (a power of two; a slice
-// accumulates in LDS).  Rows longer than max(256, mean entries per slice /
+// accumulates in LDS, 8 bytes per row for either value type).  Rows longer than max(256, mean entries per slice /
 // 128) get their own slices: in an ordinary slice such a row would put many
 // lanes of every wave on one LDS address.  Work items take about
 // nnz / (2 CUs) entries each (at least 2048), or nnz / (slices * pieces)
 // when scoo_pieces is set.
 static int plan_create_scoo_cs(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipStream_t s)
 {
-  const size_t vs = dtype_size(m->dtype);
   const uint32_t R = t.scoo_slice_size;
-  if (R == 0 || (R & (R - 1)) || R > 65536 || (uint64_t)R * vs > 163840) {
+  if (R == 0 || (R & (R - 1)) || R > 65536 || (uint64_t)R * sizeof(double) > 163840) {
     set_error("column-sorted SCOO slice_size must be a power of two whose accumulator fits in 160 KB of LDS");
     return MFP_ERR_ARG;
   }

@@ -583,7 +583,10 @@ __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *_
 
 // ---------------------------------------------------------------------------
 // Column-sorted SCOO: one workgroup per work item.  The item's slice rows
-// accumulate in LDS (ds_add); its entries come in column order, so the x
+// accumulate in LDS in double for either value type: a product of two floats
+// is exact in double, and on gfx950 ds_add_f64 sustains about 7x the lane
+// rate of ds_add_f32 (2.3 against 0.33 lanes per clock per CU with random
+// addresses), the rate that bounds this kernel.  Its entries come in column order, so the x
 // reads sweep one column range and consecutive lanes share x lines.  Lanes
 // take 4 consecutive entries at a time with 16-byte loads (two such groups
 // in flight); the unaligned head and tail go one entry per lane.  A single
@@ -620,12 +623,12 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
                           const uint32_t *__restrict__ long_mask)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *acc = reinterpret_cast<T *>(smem_raw);
+  double *acc = reinterpret_cast<double *>(smem_raw);
   const uint4 it = items[xcd_block(blockIdx.x, gridDim.x)];
   const uint32_t sl = it.x, e0 = it.y, e1 = it.z, slot = it.w;
   const uint32_t lim = sl < S_reg ? min(R, rows - sl * R) : min(R, NL - (sl - S_reg) * R);
   const uint32_t bt = blockDim.x, tid = threadIdx.x;
-  for (uint32_t i = tid; i < lim; i += bt) acc[i] = (T)0;
+  for (uint32_t i = tid; i < lim; i += bt) acc[i] = 0.0;
   __syncthreads();
   const uint32_t cbase = PACKED ? ibase[xcd_block(blockIdx.x, gridDim.x)] : 0u;
   // head: up to the first multiple of 4
@@ -633,7 +636,7 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
   for (uint32_t i = e0 + tid; i < ea; i += bt) {
     uint32_t c, lr;
     cs_entry<T, PACKED>::decode(ldg_stream(ec + i), PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
-    atomicAdd(acc + lr, ldg_stream(ev + i) * x[c]);
+    atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
   }
   // body: groups of 4 entries, two groups per lane in flight
   const uint32_t g0 = ea >> 2, g1 = e1 >> 2; // groups [g0, g1) lie inside [ea, e1)
@@ -682,7 +685,7 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
 #pragma unroll
     for (uint32_t k = 0; k < 2; k++)
 #pragma unroll
-      for (uint32_t j = 0; j < 4; j++) atomicAdd(acc + lr[k][j], v[k][j] * xv[k][j]);
+      for (uint32_t j = 0; j < 4; j++) atomicAdd(acc + lr[k][j], (double)v[k][j] * (double)xv[k][j]);
   }
   for (; g < g1; g += bt) {
     const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(ec) + g);
@@ -691,26 +694,26 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
       const uint32_t i = 4 * g + j;
       uint32_t c, lr;
       cs_entry<T, PACKED>::decode(w[j], PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
-      atomicAdd(acc + lr, ldg_stream(ev + i) * x[c]);
+      atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
     }
   }
   // tail: entries after the last whole group
   for (uint32_t i = max(ea, g1 << 2) + tid; i < e1; i += bt) {
     uint32_t c, lr;
     cs_entry<T, PACKED>::decode(ldg_stream(ec + i), PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
-    atomicAdd(acc + lr, ldg_stream(ev + i) * x[c]);
+    atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
   }
   __syncthreads();
   if (slot == CS_LONG) {
     const uint32_t *lrows = long_rows + (uint64_t)(sl - S_reg) * R;
-    for (uint32_t lr = tid; lr < lim; lr += bt) atomicAdd(y + lrows[lr], acc[lr]);
+    for (uint32_t lr = tid; lr < lim; lr += bt) atomicAdd(y + lrows[lr], (T)acc[lr]);
   } else if (slot == CS_DIRECT) {
     const uint32_t r0 = sl * R;
     for (uint32_t lr = tid; lr < lim; lr += bt)
-      if (!cs_is_long(long_mask, r0 + lr)) y[r0 + lr] = acc[lr];
+      if (!cs_is_long(long_mask, r0 + lr)) y[r0 + lr] = (T)acc[lr];
   } else {
     T *o = partial + (uint64_t)slot * R;
-    for (uint32_t lr = tid; lr < lim; lr += bt) o[lr] = acc[lr];
+    for (uint32_t lr = tid; lr < lim; lr += bt) o[lr] = (T)acc[lr];
   }
 }
 
@@ -942,7 +945,7 @@ template <typename T, bool PACKED>
 static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
                      hipStream_t s)
 {
-  const size_t lds = (size_t)l.R * sizeof(T);
+  const size_t lds = (size_t)l.R * sizeof(double);
   if (lds > 65536)
     MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T, PACKED>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
@@ -965,7 +968,7 @@ int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t t
                    hipStream_t s)
 {
   if (rows == 0 || l.NI == 0) return MFP_OK;
-  const size_t lds = (size_t)l.R * dtype_size(dtype);
+  const size_t lds = (size_t)l.R * sizeof(double);
   if (lds > 163840) {
     set_error("column-sorted SCOO slice exceeds the LDS of a workgroup");
     return MFP_ERR_ARG;

@@ -152,7 +152,7 @@ def test_scoo_colsort_within_tolerance(gpu, name, dtype, R, P):
     _tol_check(y, rp, ci, va, x)
 
 
-@pytest.mark.parametrize("dtype,R", [("float32", 32768), ("float64", 16384)])
+@pytest.mark.parametrize("dtype,R", [("float32", 16384), ("float64", 16384)])
 @pytest.mark.parametrize("P", [1, 2])
 def test_scoo_colsort_unpacked_when_columns_span_wide(gpu, dtype, R, P):
     # an item spanning >= 2^(32 - log2 R) columns keeps a separate u16 row
@@ -180,7 +180,8 @@ def test_scoo_colsort_unpacked_when_columns_span_wide(gpu, dtype, R, P):
 def test_scoo_colsort_rejects_bad_slices(gpu):
     rp, ci, va, cols = case("stencil2d_64", "float32")
     A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
-    for R in (1000, 65536):  # not a power of two; accumulator above 160 KB
+    # not a power of two; accumulators (8 bytes per row) above 160 KB
+    for R in (1000, 32768, 65536):
         with pytest.raises(gpu.MfpError):
             gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=R)
 
