@@ -630,23 +630,29 @@ __global__ void k_cs_scatter(uint64_t nnz, const uint32_t *__restrict__ perm, co
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < nnz; i += stride) {
     const uint32_t src = perm[i];
-    ev[i] = a[src];
     const uint32_t c = col[i];
+    // the item holding sorted position i: last item with e0 <= i (items
+    // are in position order and cover every position)
+    uint32_t lo = 0, hi = NI;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (items[mid].y <= i)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    // inside an aligned block of 256 positions that lies wholly in one
+    // item, position 64 j + l goes to word l, slot j: lane l's 16-byte word
+    // then holds the block's positions l, 64 + l, 128 + l and 192 + l, so
+    // each of the wave's four gathers reads 64 consecutive columns
+    const uint64_t b0 = i & ~(uint64_t)255;
+    const uint64_t d = (b0 >= items[lo].y && b0 + 256 <= items[lo].z) ? b0 + 4 * (i & 63) + ((i >> 6) & 3) : i;
+    ev[d] = a[src];
     if (packed) {
-      // the item holding sorted position i: last item with e0 <= i (items
-      // are in position order and cover every position)
-      uint32_t lo = 0, hi = NI;
-      while (hi - lo > 1) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (items[mid].y <= i)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      ec[i] = ((c - base[lo]) << rbits) | lrow[src];
+      ec[d] = ((c - base[lo]) << rbits) | lrow[src];
     } else {
-      ec[i] = c;
-      er[i] = (uint16_t)lrow[src];
+      ec[d] = c;
+      er[d] = (uint16_t)lrow[src];
     }
   }
 }
@@ -812,6 +818,41 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   if (hipStreamSynchronize(s) != hipSuccess) {
     set_error("column-sorted SCOO build failed");
     return done(MFP_ERR_HIP);
+  }
+  // Launch order: piece k of n covers about columns [k/n, (k+1)/n) of its
+  // slice, so ordering the items by the centre of that band (then by slice)
+  // gives each XCD's contiguous range of items one band of x for its L2.
+  // The scatter above needed the items in entry order; the kernel does not.
+  if (out->NI > 1) {
+    std::vector<uint32_t> hbase(out->NI);
+    if (hipMemcpyAsync(hbase.data(), out->base, (size_t)out->NI * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return done(MFP_ERR_HIP);
+    std::vector<uint32_t> pieces(out->NI), order(out->NI);
+    for (uint32_t i = 0; i < out->NI;) { // items of one slice are adjacent
+      uint32_t j = i;
+      while (j < out->NI && items[j].x == items[i].x) j++;
+      for (uint32_t k = i; k < j; k++) pieces[k] = j - i;
+      i = j;
+    }
+    std::vector<uint32_t> first(out->NI);
+    for (uint32_t i = 0; i < out->NI; i++) first[i] = (i && items[i - 1].x == items[i].x) ? first[i - 1] : i;
+    for (uint32_t i = 0; i < out->NI; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      // band centre (2k + 1) / (2n), compared exactly as a cross product
+      const uint64_t ka = a - first[a], kb = b - first[b];
+      return (2 * ka + 1) * pieces[b] < (2 * kb + 1) * pieces[a];
+    });
+    std::vector<uint4> it2(out->NI);
+    std::vector<uint32_t> b2(out->NI);
+    for (uint32_t i = 0; i < out->NI; i++) {
+      it2[i] = items[order[i]];
+      b2[i] = hbase[order[i]];
+    }
+    if (hipMemcpyAsync(out->items, it2.data(), it2.size() * sizeof(uint4), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(out->base, b2.data(), b2.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return done(MFP_ERR_HIP);
   }
   out->device_bytes = nnz * (vs + 4 + (out->packed ? 0 : 2)) + ((uint64_t)S + 1) * 4 +
                       (uint64_t)out->NI * 20 + (uint64_t)out->NF * 16 + (uint64_t)NL * 4 +

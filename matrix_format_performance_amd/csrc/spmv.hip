@@ -587,9 +587,13 @@ __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *_
 // is exact in double, and on gfx950 ds_add_f64 sustains about 7x the lane
 // rate of ds_add_f32 (2.3 against 0.33 lanes per clock per CU with random
 // addresses), the rate that bounds this kernel.  Its entries come in column order, so the x
-// reads sweep one column range and consecutive lanes share x lines.  Lanes
-// take 4 consecutive entries at a time with 16-byte loads (two such groups
-// in flight); the unaligned head and tail go one entry per lane.  A single
+// reads sweep one column range and consecutive lanes share x lines.  Each
+// aligned block of 256 positions inside an item is stored word-interleaved:
+// a lane's 16-byte load brings positions l, 64 + l, 128 + l and 192 + l, so
+// every gather instruction of a wave covers 64 consecutive sorted columns
+// (a few x lines) instead of 256 (several times as many L1 lines).  Two
+// words per lane are in flight; the unaligned head and tail go one entry
+// per lane, in position order.  A single
 // item of an ordinary slice writes y; several write partial slots that
 // k_scoo_cs_finish adds in slot order.  Items of long-row slices add into y
 // (zeroed first) with atomics; ordinary writes skip the long rows.
@@ -631,15 +635,17 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
   for (uint32_t i = tid; i < lim; i += bt) acc[i] = 0.0;
   __syncthreads();
   const uint32_t cbase = PACKED ? ibase[xcd_block(blockIdx.x, gridDim.x)] : 0u;
-  // head: up to the first multiple of 4
-  const uint32_t ea = min(e1, (e0 + 3u) & ~3u);
+  // head: up to the first multiple of 256 (positions stored in order)
+  const uint32_t ea = (uint32_t)min<uint64_t>(e1, ((uint64_t)e0 + 255u) & ~(uint64_t)255);
   for (uint32_t i = e0 + tid; i < ea; i += bt) {
     uint32_t c, lr;
     cs_entry<T, PACKED>::decode(ldg_stream(ec + i), PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
     atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
   }
-  // body: groups of 4 entries, two groups per lane in flight
-  const uint32_t g0 = ea >> 2, g1 = e1 >> 2; // groups [g0, g1) lie inside [ea, e1)
+  // body: whole blocks of 256 positions, stored so that 16-byte word l of a
+  // block holds positions l, 64 + l, 128 + l, 192 + l; two words per lane in
+  // flight.  The decode does not depend on which positions a word holds.
+  const uint32_t g0 = ea >> 2, g1 = max(g0, (e1 & ~255u) >> 2); // words [g0, g1) lie inside [ea, e1)
   uint32_t g = g0 + tid;
   for (; g + bt < g1; g += 2 * bt) {
     u32x4 w[2];
