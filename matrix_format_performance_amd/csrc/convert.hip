@@ -761,17 +761,21 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
       hipStreamSynchronize(s) != hipSuccess)
     return done(MFP_ERR_HIP);
   std::vector<uint4> items;
-  std::vector<uint4> fin; // finish blocks {slice, first partial slot, slots, first row in the slice}
+  std::vector<uint4> fin, finl; // finish blocks {slice, first partial slot, slots, first row in the slice}
   uint32_t slots = 0;
   for (uint32_t sl = 0; sl < S; sl++) {
     const uint64_t E = hsp[sl + 1] - hsp[sl];
     const uint32_t n = (uint32_t)std::max<uint64_t>(1, (E + item_entries / 2) / item_entries);
-    // long-row slices add into y with atomics; an ordinary slice with one
-    // item writes y, with several it fills partial slots
-    const uint32_t mode = sl >= S_reg ? CS_LONG : (n == 1 ? CS_DIRECT : slots);
-    if (sl < S_reg && n > 1) {
-      const uint32_t lim = std::min<uint64_t>(R, (uint64_t)rows - (uint64_t)sl * R);
+    // a slice with one item writes y; with several, each item fills a
+    // partial slot and finish blocks add the slots in slot order
+    const uint32_t mode = n == 1 ? CS_DIRECT : slots;
+    if (n > 1 && sl < S_reg) {
+      const uint32_t lim = (uint32_t)std::min<uint64_t>(R, (uint64_t)rows - (uint64_t)sl * R);
       for (uint32_t r0 = 0; r0 < lim; r0 += CS_FIN_ROWS) fin.push_back(make_uint4(sl, slots, n, r0));
+    } else if (n > 1) {
+      // a long-row slice can have many items: a wave per row adds its slots
+      const uint32_t lim = (uint32_t)std::min<uint64_t>(R, (uint64_t)NL - (uint64_t)(sl - S_reg) * R);
+      for (uint32_t r0 = 0; r0 < lim; r0 += CS_FIN_LONG) finl.push_back(make_uint4(sl, slots, n, r0));
     }
     for (uint32_t k = 0; k < n; k++) {
       const uint32_t e0 = hsp[sl] + (uint32_t)(E * k / n), e1 = hsp[sl] + (uint32_t)(E * (k + 1) / n);
@@ -779,6 +783,8 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
     }
     if (mode < CS_DIRECT) slots += n;
   }
+  out->NFR = (uint32_t)fin.size();
+  fin.insert(fin.end(), finl.begin(), finl.end());
   out->NI = (uint32_t)items.size();
   out->NF = (uint32_t)fin.size();
   out->slots = slots;

@@ -94,13 +94,15 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_
 // Slices 0..S_reg-1 hold R consecutive rows each; slices S_reg..S-1 hold the
 // long rows, R per slice in row order (long_rows[] gives their rows,
 // long_mask[] marks them).  A work item {slice, first entry, end entry, mode}
-// is one workgroup; mode is a partial slot, CS_DIRECT (the item writes the
-// slice's rows of y) or CS_LONG (it adds into y with atomics).  fin[] lists
-// finish blocks {slice, first slot, slots, first row} of CS_FIN_ROWS rows.
-constexpr uint32_t CS_DIRECT = 0xFFFFFFFEu, CS_LONG = 0xFFFFFFFFu, CS_FIN_ROWS = 256;
+// is one workgroup; mode is a partial slot, or CS_DIRECT when the item is its
+// slice's only one and writes the slice's rows of y.  fin[] lists finish
+// blocks {slice, first slot, slots, first row}: the first NFR cover
+// CS_FIN_ROWS rows of an ordinary slice (a thread per row), the rest
+// CS_FIN_LONG rows of a long-row slice (a wave per row, lanes over slots).
+constexpr uint32_t CS_DIRECT = 0xFFFFFFFEu, CS_FIN_ROWS = 256, CS_FIN_LONG = CS_FIN_ROWS / 64;
 struct scoo_cs_layout {
   uint32_t R = 0, rbits = 0, S_reg = 0, S_long = 0, S = 0, NL = 0;
-  uint32_t NI = 0, NF = 0, slots = 0;
+  uint32_t NI = 0, NF = 0, NFR = 0, slots = 0;
   int packed = 0;
   uint32_t *sp = nullptr;        // S + 1 entry offsets of the slices
   uint32_t *long_rows = nullptr; // NL

@@ -593,10 +593,12 @@ __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *_
 // every gather instruction of a wave covers 64 consecutive sorted columns
 // (a few x lines) instead of 256 (several times as many L1 lines).  Two
 // words per lane are in flight; the unaligned head and tail go one entry
-// per lane, in position order.  A single
-// item of an ordinary slice writes y; several write partial slots that
-// k_scoo_cs_finish adds in slot order.  Items of long-row slices add into y
-// (zeroed first) with atomics; ordinary writes skip the long rows.
+// per lane, in position order.  A slice's
+// single item writes y; several write partial slots that k_scoo_cs_finish
+// adds in slot order.  Long-row slices work the same way, their local rows
+// mapped through long_rows; ordinary slices skip the long rows.  Each y entry
+// is written once, with no global atomics (the order of the LDS additions
+// still varies from run to run).
 // ---------------------------------------------------------------------------
 template <typename T, bool PACKED>
 struct cs_entry {
@@ -710,9 +712,9 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
     atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
   }
   __syncthreads();
-  if (slot == CS_LONG) {
+  if (slot == CS_DIRECT && sl >= S_reg) {
     const uint32_t *lrows = long_rows + (uint64_t)(sl - S_reg) * R;
-    for (uint32_t lr = tid; lr < lim; lr += bt) atomicAdd(y + lrows[lr], (T)acc[lr]);
+    for (uint32_t lr = tid; lr < lim; lr += bt) y[lrows[lr]] = (T)acc[lr];
   } else if (slot == CS_DIRECT) {
     const uint32_t r0 = sl * R;
     for (uint32_t lr = tid; lr < lim; lr += bt)
@@ -723,22 +725,30 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
   }
 }
 
+// One workgroup per finish block.  Blocks below NFR: CS_FIN_ROWS rows of an
+// ordinary slice, a thread per row, y[row] = the slice's slots added in slot
+// order (long rows skipped).  The others: CS_FIN_LONG rows of a long-row
+// slice, a wave per row; lane l adds slots l, l + 64, ... in order and the
+// wave's halving tree adds the lanes, into y[long_rows[(slice - S_reg) R + lr]].
 template <typename T>
-__global__ void k_scoo_cs_zero_long(uint32_t NL, const uint32_t *__restrict__ long_rows, T *__restrict__ y)
-{
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < NL) y[long_rows[j]] = (T)0;
-}
-
-// one workgroup per finish block of CS_FIN_ROWS rows: y[row] = the sum of the
-// slice's slots, in slot order (long rows are left to their atomics)
-template <typename T>
-__global__ void k_scoo_cs_finish(uint32_t rows, uint32_t R, const uint4 *__restrict__ fin,
-                                 const uint32_t *__restrict__ long_mask, const T *__restrict__ partial,
-                                 T *__restrict__ y)
+__global__ __launch_bounds__(CS_FIN_ROWS) void k_scoo_cs_finish(uint32_t rows, uint32_t R, uint32_t S_reg,
+                                                                uint32_t NL, uint32_t NFR,
+                                                                const uint4 *__restrict__ fin,
+                                                                const uint32_t *__restrict__ long_rows,
+                                                                const uint32_t *__restrict__ long_mask,
+                                                                const T *__restrict__ partial, T *__restrict__ y)
 {
   const uint4 f = fin[blockIdx.x];
   const uint32_t sl = f.x, slot0 = f.y, n = f.z;
+  if (blockIdx.x >= NFR) {
+    const uint32_t lr = f.w + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (lr >= min(R, NL - (sl - S_reg) * R)) return; // uniform across the wave
+    T d = (T)0;
+    for (uint32_t k = lane; k < n; k += 64) d = d + partial[(uint64_t)(slot0 + k) * R + lr];
+    d = group_reduce<64>(d);
+    if (lane == 0) y[long_rows[(uint64_t)(sl - S_reg) * R + lr]] = d;
+    return;
+  }
   const uint32_t lim = min(R, rows - sl * R);
   const uint32_t lr = f.w + threadIdx.x;
   if (lr >= lim || threadIdx.x >= CS_FIN_ROWS) return;
@@ -955,16 +965,13 @@ static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, c
   if (lds > 65536)
     MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T, PACKED>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-  if (l.NL) {
-    k_scoo_cs_zero_long<T><<<ceil_div_u32(l.NL, 256), 256, 0, s>>>(l.NL, l.long_rows, (T *)y);
-    MFP_CHECK(launched());
-  }
   k_scoo_cs<T, PACKED><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
                                                   (const T *)l.ev, l.ec, l.er, (const T *)x, (T *)y,
                                                   (T *)l.partial, l.long_rows, l.long_mask);
   MFP_CHECK(launched());
   if (l.NF) {
-    k_scoo_cs_finish<T><<<l.NF, CS_FIN_ROWS, 0, s>>>(rows, l.R, l.fin, l.long_mask, (const T *)l.partial, (T *)y);
+    k_scoo_cs_finish<T><<<l.NF, CS_FIN_ROWS, 0, s>>>(rows, l.R, l.S_reg, l.NL, l.NFR, l.fin, l.long_rows,
+                                                      l.long_mask, (const T *)l.partial, (T *)y);
     MFP_CHECK(launched());
   }
   return MFP_OK;
