@@ -174,10 +174,11 @@ enum {
   MFP_TUNE_IS_DEFAULT = 2,  /* set by the autotuner on each format's default candidate; no effect */
   /* SCOO with each slice's entries sorted by column and split into work
    * items (about scoo_pieces per slice; 0 = about two per CU over the whole
-   * matrix); scoo_slice_size rows (a power of two) accumulate in LDS, and
-   * rows much longer than a slice's mean share get slices of their own.  No
-   * reference counterpart (the reference's SCOO keeps row order,
-   * scoo_matrix_class.cpp:102). */
+   * matrix), never across a column band of 2^(32 - log2 scoo_slice_size)
+   * columns; scoo_slice_size rows (a power of two, at most 16384: 8 bytes
+   * of LDS per row) accumulate in LDS, and rows much longer than a slice's
+   * mean share get slices of their own.  No reference counterpart (the
+   * reference's SCOO keeps row order, scoo_matrix_class.cpp:102). */
   MFP_TUNE_SCOO_COLSORT = 4
 };
 
@@ -231,7 +232,7 @@ typedef struct {
   uint32_t column_bytes;     /* bytes per stored column index (2 with 16-bit offsets) */
   uint32_t scoo_colsort;     /* column-sorted SCOO: slices_count counts the long-row slices too,
                                 lane_size holds the work items */
-  uint32_t scoo_packed;      /* column-sorted SCOO: column offset and row packed in one u32 */
+  uint32_t scoo_packed;      /* column-sorted SCOO: 1 (each entry packs its column offset and row in one u32) */
   uint32_t scoo_long_rows;   /* column-sorted SCOO: rows moved to long-row slices */
 } mfp_plan_info;
 int mfp_plan_get_info(mfp_plan p, mfp_plan_info *out);

@@ -3,8 +3,8 @@
 // Swept parameters: lanes per row (CSR-vector, 2..64), CSR-Adaptive block
 // size, ELL width (longest row), the ELL/COO split of the hybrids (row-length
 // quantiles), the SCOO slice size (lane count filling a 32 KB LDS
-// accumulator) and column-sorted SCOO slices of 4096..16384 rows (whole or
-// in 2 or 4 pieces per slice).  Every candidate is built as a plan and timed with an event
+// accumulator) and column-sorted SCOO slices of 4096..16384 rows (pieces per
+// slice and workgroup size).  Every candidate is built as a plan and timed with an event
 // pair (median of reps, each after an Infinity-Cache flush when asked).
 #include "../../include/mfp.h"
 
@@ -103,16 +103,20 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
         push(c, f, t);
       }
       // column-sorted slices (8-byte LDS accumulators): pieces chosen by
-      // the plan, or 2 and 4 equal pieces per slice
+      // the plan, or 2 and 4 pieces per slice; a 16384-row slice fills
+      // 128 KB of LDS (one workgroup per CU), so it also tries 1024 threads
       for (uint32_t R : {4096u, 8192u, 16384u}) {
         if (v.rows && R > 4096 && R / 2 >= v.rows) break;
-        for (uint32_t P : {0u, 2u, 4u}) {
-          mfp_tuning tc = def;
-          tc.flags = MFP_TUNE_SCOO_COLSORT;
-          tc.scoo_slice_size = R;
-          tc.scoo_pieces = P;
-          push(c, f, tc);
-        }
+        for (uint32_t P : {0u, 2u, 4u})
+          for (uint32_t th : {512u, 1024u}) {
+            if (th == 1024 && R < 16384) continue;
+            mfp_tuning tc = def;
+            tc.flags = MFP_TUNE_SCOO_COLSORT;
+            tc.scoo_slice_size = R;
+            tc.scoo_pieces = P;
+            tc.scoo_threads = th;
+            push(c, f, tc);
+          }
       }
       break;
     }

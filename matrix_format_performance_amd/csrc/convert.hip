@@ -590,6 +590,26 @@ __global__ void k_entry_slice(uint32_t rows, uint64_t nnz, const uint32_t *__res
 }
 
 // sp[s] = first sorted position whose slice is >= s
+// bb[sl (nb + 1) + k] = first entry of slice sl whose column is >= k W
+// (entries of a slice are sorted by column), for k = 0..nb
+__global__ void k_band_bounds(uint32_t S, uint32_t nb, uint64_t W, const uint32_t *__restrict__ sp,
+                              const uint32_t *__restrict__ col_sorted, uint32_t *__restrict__ bb)
+{
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)S * (nb + 1)) return;
+  const uint32_t sl = (uint32_t)(t / (nb + 1)), k = (uint32_t)(t % (nb + 1));
+  uint32_t lo = sp[sl], hi = sp[sl + 1];
+  const uint64_t c0 = (uint64_t)k * W;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if ((uint64_t)col_sorted[mid] < c0)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  bb[t] = lo;
+}
+
 __global__ void k_slice_bounds(uint32_t S, uint64_t nnz, const uint32_t *__restrict__ slice_sorted,
                                uint32_t *__restrict__ sp)
 {
@@ -624,7 +644,7 @@ template <typename T>
 __global__ void k_cs_scatter(uint64_t nnz, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ col,
                              const uint32_t *__restrict__ lrow, const T *__restrict__ a, uint32_t NI,
                              const uint4 *__restrict__ items, const uint32_t *__restrict__ base, uint32_t rbits,
-                             int packed, T *__restrict__ ev, uint32_t *__restrict__ ec, uint16_t *__restrict__ er)
+                             T *__restrict__ ev, uint32_t *__restrict__ ec)
 {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -648,22 +668,17 @@ __global__ void k_cs_scatter(uint64_t nnz, const uint32_t *__restrict__ perm, co
     const uint64_t b0 = i & ~(uint64_t)255;
     const uint64_t d = (b0 >= items[lo].y && b0 + 256 <= items[lo].z) ? b0 + 4 * (i & 63) + ((i >> 6) & 3) : i;
     ev[d] = a[src];
-    if (packed) {
-      ec[d] = ((c - base[lo]) << rbits) | lrow[src];
-    } else {
-      ec[d] = c;
-      er[d] = (uint16_t)lrow[src];
-    }
+    ec[d] = ((c - base[lo]) << rbits) | lrow[src];
   }
 }
 
 template <typename T>
 static int cs_scatter_t(uint64_t nnz, const uint32_t *perm, const uint32_t *col, const uint32_t *lrow,
                         const void *a, uint32_t NI, const uint4 *items, const uint32_t *base, uint32_t rbits,
-                        int packed, void *ev, uint32_t *ec, uint16_t *er, hipStream_t s)
+                        void *ev, uint32_t *ec, hipStream_t s)
 {
-  k_cs_scatter<T><<<grid_for(nnz), 256, 0, s>>>(nnz, perm, col, lrow, (const T *)a, NI, items, base, rbits, packed,
-                                                (T *)ev, ec, er);
+  k_cs_scatter<T><<<grid_for(nnz), 256, 0, s>>>(nnz, perm, col, lrow, (const T *)a, NI, items, base, rbits,
+                                                (T *)ev, ec);
   MFP_HIP(hipGetLastError());
   return MFP_OK;
 }
@@ -760,12 +775,43 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   if (hipMemcpyAsync(hsp.data(), out->sp, hsp.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return done(MFP_ERR_HIP);
+  // Column bands of width W = 2^(32 - rbits): an item inside one band packs
+  // (column - item base) and the local row into one u32.  Each non-empty
+  // band of a slice is cut into about E_band / item_entries equal pieces.
+  const uint64_t W = rbits ? (1ull << (32 - rbits)) : (1ull << 32);
+  const uint32_t nb = (uint32_t)std::max<uint64_t>(1, ((uint64_t)cols + W - 1) / W);
+  std::vector<uint32_t> hbb((size_t)S * (nb + 1));
+  if (S && nb > 1) {
+    uint32_t *bb = nullptr;
+    st = scratch((void **)&bb, hbb.size() * 4);
+    if (st != MFP_OK) return done(st);
+    k_band_bounds<<<ceil_div_u32(hbb.size(), 256), 256, 0, s>>>(S, nb, W, out->sp, cur[1], bb);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(hbb.data(), bb, hbb.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return done(MFP_ERR_HIP);
+  } else {
+    for (uint32_t sl = 0; sl < S; sl++) {
+      hbb[(size_t)sl * 2] = hsp[sl];
+      hbb[(size_t)sl * 2 + 1] = hsp[sl + 1];
+    }
+  }
   std::vector<uint4> items;
   std::vector<uint4> fin, finl; // finish blocks {slice, first partial slot, slots, first row in the slice}
   uint32_t slots = 0;
+  std::vector<std::pair<uint32_t, uint32_t>> cuts; // [e0, e1) of the slice's items
   for (uint32_t sl = 0; sl < S; sl++) {
-    const uint64_t E = hsp[sl + 1] - hsp[sl];
-    const uint32_t n = (uint32_t)std::max<uint64_t>(1, (E + item_entries / 2) / item_entries);
+    cuts.clear();
+    for (uint32_t k = 0; k < nb; k++) {
+      const uint32_t b0 = hbb[(size_t)sl * (nb + 1) + k], b1 = hbb[(size_t)sl * (nb + 1) + k + 1];
+      const uint64_t Eb = b1 - b0;
+      if (Eb == 0) continue;
+      const uint64_t nk = std::max<uint64_t>(1, (Eb + item_entries / 2) / item_entries);
+      for (uint64_t q = 0; q < nk; q++)
+        cuts.push_back({b0 + (uint32_t)(Eb * q / nk), b0 + (uint32_t)(Eb * (q + 1) / nk)});
+    }
+    if (cuts.empty()) cuts.push_back({hsp[sl], hsp[sl]}); // an empty slice still writes its zeros
+    const uint32_t n = (uint32_t)cuts.size();
     // a slice with one item writes y; with several, each item fills a
     // partial slot and finish blocks add the slots in slot order
     const uint32_t mode = n == 1 ? CS_DIRECT : slots;
@@ -777,10 +823,8 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
       const uint32_t lim = (uint32_t)std::min<uint64_t>(R, (uint64_t)NL - (uint64_t)(sl - S_reg) * R);
       for (uint32_t r0 = 0; r0 < lim; r0 += CS_FIN_LONG) finl.push_back(make_uint4(sl, slots, n, r0));
     }
-    for (uint32_t k = 0; k < n; k++) {
-      const uint32_t e0 = hsp[sl] + (uint32_t)(E * k / n), e1 = hsp[sl] + (uint32_t)(E * (k + 1) / n);
-      items.push_back(make_uint4(sl, e0, e1, mode < CS_DIRECT ? mode + k : mode));
-    }
+    for (uint32_t k = 0; k < n; k++)
+      items.push_back(make_uint4(sl, cuts[k].first, cuts[k].second, mode < CS_DIRECT ? mode + k : mode));
     if (mode < CS_DIRECT) slots += n;
   }
   out->NFR = (uint32_t)fin.size();
@@ -810,15 +854,14 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   uint32_t hspan = 0;
   if (hipMemcpyAsync(&hspan, span, 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
     return done(MFP_ERR_HIP);
-  out->packed = rbits == 0 || (uint64_t)hspan < (1ull << (32 - rbits));
-  if (!out->packed) {
-    st = dmalloc((void **)&out->er, std::max<uint64_t>(nnz, 1) * 2);
-    if (st != MFP_OK) return done(st);
+  // every item lies inside one column band, so its offsets fit beside the row
+  if (rbits && (uint64_t)hspan >= (1ull << (32 - rbits))) {
+    set_error("column-sorted SCOO: an item spans more than its column band");
+    return done(MFP_ERR_STATE);
   }
   if (nnz) {
     st = (dtype == MFP_F32 ? cs_scatter_t<float> : cs_scatter_t<double>)(
-        nnz, cur[2], cur[1], lrow, val, out->NI, out->items, out->base, rbits, out->packed, out->ev, out->ec, out->er,
-        s);
+        nnz, cur[2], cur[1], lrow, val, out->NI, out->items, out->base, rbits, out->ev, out->ec, s);
     if (st != MFP_OK) return done(st);
   }
   if (hipStreamSynchronize(s) != hipSuccess) {
@@ -860,7 +903,7 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
         hipStreamSynchronize(s) != hipSuccess)
       return done(MFP_ERR_HIP);
   }
-  out->device_bytes = nnz * (vs + 4 + (out->packed ? 0 : 2)) + ((uint64_t)S + 1) * 4 +
+  out->device_bytes = nnz * (vs + 4) + ((uint64_t)S + 1) * 4 +
                       (uint64_t)out->NI * 20 + (uint64_t)out->NF * 16 + (uint64_t)NL * 4 +
                       (NL ? ((uint64_t)rows + 31) / 32 * 4 : 0) + (uint64_t)slots * R * vs;
   return done(MFP_OK);
@@ -877,7 +920,6 @@ void scoo_cs_free(scoo_cs_layout *l)
   dfree(l->partial);
   dfree(l->ev);
   dfree(l->ec);
-  dfree(l->er);
   *l = scoo_cs_layout{};
 }
 

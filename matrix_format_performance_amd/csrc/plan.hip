@@ -632,7 +632,7 @@ int mfp_plan_get_info(mfp_plan p, mfp_plan_info *o)
   o->nnz_per_wg = p->t.nnz_per_wg;
   o->column_bytes = p->off16 ? 2u : 4u;
   o->scoo_colsort = p->colsort ? 1u : 0u;
-  o->scoo_packed = p->colsort && p->cs.packed ? 1u : 0u;
+  o->scoo_packed = p->colsort ? 1u : 0u;
   o->scoo_long_rows = p->colsort ? p->cs.NL : 0u;
   if (p->colsort) o->stored_elements = p->m->nnz;
   return MFP_OK;

@@ -600,31 +600,23 @@ __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *_
 // is written once, with no global atomics (the order of the LDS additions
 // still varies from run to run).
 // ---------------------------------------------------------------------------
-template <typename T, bool PACKED>
-struct cs_entry {
-  __device__ __forceinline__ static void decode(uint32_t w, uint32_t r16, uint32_t cbase, uint32_t rbits,
-                                                uint32_t &c, uint32_t &lr)
-  {
-    if constexpr (PACKED) {
-      c = cbase + (w >> rbits);
-      lr = w & ((1u << rbits) - 1u);
-    } else {
-      c = w;
-      lr = r16;
-    }
-  }
-};
+// entry word: (column - item base) << rbits | local row
+__device__ __forceinline__ void cs_decode(uint32_t w, uint32_t cbase, uint32_t rbits, uint32_t &c, uint32_t &lr)
+{
+  c = cbase + (w >> rbits);
+  lr = w & ((1u << rbits) - 1u);
+}
 
 __device__ __forceinline__ bool cs_is_long(const uint32_t *__restrict__ mask, uint32_t row)
 {
   return mask && ((mask[row >> 5] >> (row & 31u)) & 1u);
 }
 
-template <typename T, bool PACKED>
+template <typename T>
 __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_reg, uint32_t NL,
                           const uint4 *__restrict__ items, const uint32_t *__restrict__ ibase,
                           const T *__restrict__ ev, const uint32_t *__restrict__ ec,
-                          const uint16_t *__restrict__ er, const T *__restrict__ x, T *__restrict__ y,
+                          const T *__restrict__ x, T *__restrict__ y,
                           T *__restrict__ partial, const uint32_t *__restrict__ long_rows,
                           const uint32_t *__restrict__ long_mask)
 {
@@ -636,12 +628,12 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
   const uint32_t bt = blockDim.x, tid = threadIdx.x;
   for (uint32_t i = tid; i < lim; i += bt) acc[i] = 0.0;
   __syncthreads();
-  const uint32_t cbase = PACKED ? ibase[xcd_block(blockIdx.x, gridDim.x)] : 0u;
+  const uint32_t cbase = ibase[xcd_block(blockIdx.x, gridDim.x)];
   // head: up to the first multiple of 256 (positions stored in order)
   const uint32_t ea = (uint32_t)min<uint64_t>(e1, ((uint64_t)e0 + 255u) & ~(uint64_t)255);
   for (uint32_t i = e0 + tid; i < ea; i += bt) {
     uint32_t c, lr;
-    cs_entry<T, PACKED>::decode(ldg_stream(ec + i), PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
+    cs_decode(ldg_stream(ec + i), cbase, rbits, c, lr);
     atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
   }
   // body: whole blocks of 256 positions, stored so that 16-byte word l of a
@@ -652,7 +644,6 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
   for (; g + bt < g1; g += 2 * bt) {
     u32x4 w[2];
     T v[2][4];
-    uint32_t r16[2][4];
 #pragma unroll
     for (uint32_t k = 0; k < 2; k++) {
       const uint32_t gg = g + k * bt;
@@ -669,23 +660,13 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
         v[k][2] = v1[0];
         v[k][3] = v1[1];
       }
-      if constexpr (!PACKED) {
-        const u32x2 rr = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(er) + gg);
-        r16[k][0] = rr[0] & 0xFFFFu;
-        r16[k][1] = rr[0] >> 16;
-        r16[k][2] = rr[1] & 0xFFFFu;
-        r16[k][3] = rr[1] >> 16;
-      } else {
-#pragma unroll
-        for (uint32_t j = 0; j < 4; j++) r16[k][j] = 0;
-      }
     }
     uint32_t c[2][4], lr[2][4];
     T xv[2][4];
 #pragma unroll
     for (uint32_t k = 0; k < 2; k++)
 #pragma unroll
-      for (uint32_t j = 0; j < 4; j++) cs_entry<T, PACKED>::decode(w[k][j], r16[k][j], cbase, rbits, c[k][j], lr[k][j]);
+      for (uint32_t j = 0; j < 4; j++) cs_decode(w[k][j], cbase, rbits, c[k][j], lr[k][j]);
 #pragma unroll
     for (uint32_t k = 0; k < 2; k++)
 #pragma unroll
@@ -701,14 +682,14 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
     for (uint32_t j = 0; j < 4; j++) {
       const uint32_t i = 4 * g + j;
       uint32_t c, lr;
-      cs_entry<T, PACKED>::decode(w[j], PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
+      cs_decode(w[j], cbase, rbits, c, lr);
       atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
     }
   }
   // tail: entries after the last whole group
   for (uint32_t i = max(ea, g1 << 2) + tid; i < e1; i += bt) {
     uint32_t c, lr;
-    cs_entry<T, PACKED>::decode(ldg_stream(ec + i), PACKED ? 0u : ldg_stream(er + i), cbase, rbits, c, lr);
+    cs_decode(ldg_stream(ec + i), cbase, rbits, c, lr);
     atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
   }
   __syncthreads();
@@ -957,16 +938,16 @@ int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t thre
   return launched();
 }
 
-template <typename T, bool PACKED>
+template <typename T>
 static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
                      hipStream_t s)
 {
   const size_t lds = (size_t)l.R * sizeof(double);
   if (lds > 65536)
-    MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T, PACKED>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-  k_scoo_cs<T, PACKED><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
-                                                  (const T *)l.ev, l.ec, l.er, (const T *)x, (T *)y,
+  k_scoo_cs<T><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
+                                                  (const T *)l.ev, l.ec, (const T *)x, (T *)y,
                                                   (T *)l.partial, l.long_rows, l.long_mask);
   MFP_CHECK(launched());
   if (l.NF) {
@@ -990,11 +971,8 @@ int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t t
     set_error("SCOO workgroup size must be a multiple of 64 up to 1024");
     return MFP_ERR_ARG;
   }
-  if (dtype == MFP_F32)
-    return l.packed ? scoo_cs_t<float, true>(rows, l, threads, x, y, s)
-                    : scoo_cs_t<float, false>(rows, l, threads, x, y, s);
-  return l.packed ? scoo_cs_t<double, true>(rows, l, threads, x, y, s)
-                  : scoo_cs_t<double, false>(rows, l, threads, x, y, s);
+  if (dtype == MFP_F32) return scoo_cs_t<float>(rows, l, threads, x, y, s);
+  return scoo_cs_t<double>(rows, l, threads, x, y, s);
 }
 
 } // namespace mfp

@@ -152,29 +152,29 @@ def test_scoo_colsort_within_tolerance(gpu, name, dtype, R, P):
     _tol_check(y, rp, ci, va, x)
 
 
-@pytest.mark.parametrize("dtype,R", [("float32", 16384), ("float64", 16384)])
+@pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("P", [1, 2])
-def test_scoo_colsort_unpacked_when_columns_span_wide(gpu, dtype, R, P):
-    # an item spanning >= 2^(32 - log2 R) columns keeps a separate u16 row
+def test_scoo_colsort_wide_columns_cut_at_bands(gpu, dtype, P):
+    # 16384-row slices leave 18 bits for the column offset: the 300,000
+    # columns make two bands of 262,144, and no item may straddle them
     rp, ci, va = gpu.generate(gpu.GenSpec("uniform", values="uniform", rows=300000, nnz_lo=1, nnz_hi=5, seed=5),
                               np.dtype(dtype))
     cols = rp.size - 1
-    announce(f"scoo colsort unpacked dtype={dtype} R={R} P={P}")
+    R = 16384
+    announce(f"scoo colsort bands dtype={dtype} R={R} P={P}")
     x = gpu.generate_vector(3, cols, np.dtype(dtype))
     A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
     plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=R, scoo_pieces=P)
     info = plan.info()
-    if P == 1:
-        assert info["scoo_packed"] == 0  # one item per slice spans nearly all 300,000 columns
+    assert info["scoo_packed"] == 1
+    S = -(-cols // R)
+    assert info["lane_size"] >= 2 * S  # every slice has entries in both bands
     xd = gpu.DeviceArray.from_host(x)
     yd = gpu.DeviceArray(cols, np.dtype(dtype))
     yd.fill(np.nan)
     plan.spmv(xd, yd)
     gpu.synchronize()
     _tol_check(yd.to_host(), rp, ci, va, x)
-    # a narrower slice packs
-    p2 = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=1024, scoo_pieces=P)
-    assert p2.info()["scoo_packed"] == 1
 
 
 def test_scoo_colsort_rejects_bad_slices(gpu):
