@@ -92,7 +92,7 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
     }
     case MFP_SCOO: {
       push(c, f, tagged);
-      const uint32_t acc = (v.dtype == MFP_F64 ? 4096u : 8192u);
+      const uint32_t acc = 4096u; // 32 KB of double accumulators
       for (uint32_t ss : {256u, 512u, 1024u, 2048u, 4096u}) {
         if (v.rows && ss > std::max<uint32_t>(256, v.rows)) break;
         t.scoo_slice_size = ss;

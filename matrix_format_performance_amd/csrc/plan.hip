@@ -121,13 +121,15 @@ static uint32_t prev_pow2_u32(uint32_t v)
 }
 
 // MI355X SCOO sizing: a slice's accumulator acc[slice_size][lane_size] takes
-// 32 KB of the 160 KB LDS (4 resident workgroups per CU), with at least
-// 4 slices per CU when the row count allows.
+// 32 KB of the 160 KB LDS (4 resident workgroups per CU) in double, the type
+// the kernel accumulates FP32 values in too, with at least 4 slices per CU
+// when the row count allows.
 static void scoo_mi355x_geometry(int dtype, uint32_t rows, uint32_t *slices, uint32_t *ss,
                                  uint32_t *L)
 {
+  (void)dtype;
   const uint32_t cus = (uint32_t)std::max(1, props().cu_count);
-  const uint32_t acc_elems = (uint32_t)((32u * 1024u) / dtype_size(dtype));
+  const uint32_t acc_elems = (uint32_t)((32u * 1024u) / sizeof(double));
   uint32_t s = prev_pow2_u32(std::max<uint32_t>(1, rows / (4 * cus)));
   s = std::min<uint32_t>(std::max<uint32_t>(s, 64), acc_elems / 2);
   uint32_t lanes = std::min<uint32_t>(32, prev_pow2_u32(acc_elems / s));

@@ -534,21 +534,24 @@ __global__ __launch_bounds__(256) void k_hyb_atomic(uint32_t rows, uint32_t w,
 // ---------------------------------------------------------------------------
 // SCOO (scoo_spmv.cu:46-129): one workgroup per slice of slice_size rows.
 // Products are added with LDS atomics (ds_add) into acc[row][lane], lane =
-// thread % lane_size, then each row's lanes are summed in order.
+// thread % lane_size, then each row's lanes are summed in order.  FP32
+// values accumulate in double (A) when acc[slice_size][lane_size] fits in
+// LDS that way: the float product is exact in double, and gfx950's
+// ds_add_f64 sustains several times the lane rate of ds_add_f32.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, typename A>
 __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *__restrict__ cr,
                        const uint32_t *__restrict__ cc, const T *__restrict__ cv,
                        const uint32_t *__restrict__ sp, const T *__restrict__ x,
                        T *__restrict__ y)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T *acc = reinterpret_cast<T *>(smem_raw);
+  A *acc = reinterpret_cast<A *>(smem_raw);
   const uint32_t s = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t r0 = s * ss;
   const uint32_t lim = min(ss, rows - r0);
   const uint32_t bt = blockDim.x, tid = threadIdx.x;
-  for (uint32_t i = tid; i < lim * L; i += bt) acc[i] = (T)0;
+  for (uint32_t i = tid; i < lim * L; i += bt) acc[i] = (A)0;
   __syncthreads();
   const uint32_t b = sp[s], e = sp[s + 1];
   const uint32_t lane = tid & (L - 1u);
@@ -566,18 +569,18 @@ __global__ void k_scoo(uint32_t rows, uint32_t ss, uint32_t L, const uint32_t *_
     for (uint32_t k = 0; k < 4; k++) xv[k] = x[c[k]];
 #pragma unroll
     for (uint32_t k = 0; k < 4; k++)
-      if (lr[k] < lim) atomicAdd(acc + lr[k] * L + lane, v[k] * xv[k]);
+      if (lr[k] < lim) atomicAdd(acc + lr[k] * L + lane, (A)v[k] * (A)xv[k]);
   }
   for (; i < e; i += bt) {
     const uint32_t lr = ldg_stream(cr + i) - r0;
-    const T p = ldg_stream(cv + i) * x[ldg_stream(cc + i)];
+    const A p = (A)ldg_stream(cv + i) * (A)x[ldg_stream(cc + i)];
     if (lr < lim) atomicAdd(acc + lr * L + lane, p);
   }
   __syncthreads();
   for (uint32_t lr = tid; lr < lim; lr += bt) {
-    T d = 0;
+    A d = 0;
     for (uint32_t l = 0; l < L; l++) d = d + acc[lr * L + l];
-    y[r0 + lr] = d;
+    y[r0 + lr] = (T)d;
   }
 }
 
@@ -928,13 +931,19 @@ int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t thre
     return MFP_ERR_ARG;
   }
   const uint32_t real = ceil_div_u32(rows, ss);
-  if (dtype == MFP_F32) {
-    if (lds > 65536) MFP_HIP(hipFuncSetAttribute((const void *)k_scoo<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_scoo<float><<<real, threads, lds, s>>>(rows, ss, L, cr, cc, (const float *)cv, sp, (const float *)x, (float *)y);
-  } else {
-    if (lds > 65536) MFP_HIP(hipFuncSetAttribute((const void *)k_scoo<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_scoo<double><<<real, threads, lds, s>>>(rows, ss, L, cr, cc, (const double *)cv, sp, (const double *)x, (double *)y);
-  }
+  auto go = [&](auto kern, size_t bytes, auto tv) {
+    using TV = decltype(tv);
+    if (bytes > 65536)
+      MFP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    kern<<<real, threads, bytes, s>>>(rows, ss, L, cr, cc, (const TV *)cv, sp, (const TV *)x, (TV *)y);
+    return MFP_OK;
+  };
+  if (dtype == MFP_F64)
+    MFP_CHECK(go(k_scoo<double, double>, lds, 0.0));
+  else if (2 * lds <= 160u * 1024u)
+    MFP_CHECK(go(k_scoo<float, double>, 2 * lds, 0.0f));
+  else
+    MFP_CHECK(go(k_scoo<float, float>, lds, 0.0f));
   return launched();
 }
 
