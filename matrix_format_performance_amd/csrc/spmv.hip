@@ -380,6 +380,23 @@ __global__ __launch_bounds__(AD_T) void k_csr_adaptive_persistent(
 }
 
 // One lane per long row: add its pieces left to right.
+// p[0] + p[stride] + ... + p[(n - 1) stride], added left to right (n >= 1).
+// Loads go out eight at a time with clamped indices, so none waits on the
+// sum before it is issued; the additions keep the order.
+template <typename T>
+__device__ __forceinline__ T sum_in_order(const T *__restrict__ p, uint32_t n, uint64_t stride)
+{
+  T d = p[0];
+  for (uint32_t k = 1; k < n; k += 8) {
+    T q[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) q[j] = p[(uint64_t)min(k + j, n - 1) * stride];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) d = (k + j < n) ? d + q[j] : d;
+  }
+  return d;
+}
+
 template <typename T>
 __global__ void k_adaptive_finish(const uint4 *__restrict__ chunks, uint32_t nchunks,
                                   const T *__restrict__ partials, T *__restrict__ y)
@@ -388,9 +405,7 @@ __global__ void k_adaptive_finish(const uint4 *__restrict__ chunks, uint32_t nch
   if (j >= nchunks) return;
   const uint4 c4 = chunks[j];
   if (c4.y != 0) return;
-  T acc = partials[c4.z];
-  for (uint32_t i = 1; i < c4.w; i++) acc = acc + partials[c4.z + i];
-  y[c4.x] = acc;
+  y[c4.x] = sum_in_order(partials + c4.z, c4.w, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -738,9 +753,7 @@ __global__ __launch_bounds__(CS_FIN_ROWS) void k_scoo_cs_finish(uint32_t rows, u
   if (lr >= lim || threadIdx.x >= CS_FIN_ROWS) return;
   const uint32_t row = sl * R + lr;
   if (cs_is_long(long_mask, row)) return;
-  T d = partial[(uint64_t)slot0 * R + lr];
-  for (uint32_t k = 1; k < n; k++) d = d + partial[(uint64_t)(slot0 + k) * R + lr];
-  y[row] = d;
+  y[row] = sum_in_order(partial + (uint64_t)slot0 * R + lr, n, R);
 }
 
 // ---------------------------------------------------------------------------
