@@ -253,7 +253,8 @@ def tuning_label(fmt: str, t: dict, marks: dict, dtype=np.float32) -> str:
         return {base // 2: "W/2", base: "W", base * 2: "2W"}.get(t["nnz_per_wg"], f"W{t['nnz_per_wg']}")
     if fmt == "scoo":
         if t["flags"] & L.MFP_TUNE_SCOO_COLSORT:
-            return f"cs{t['scoo_slice_size']}"
+            th = "t1024" if t.get("scoo_threads") == 1024 else ""
+            return f"cs{t['scoo_slice_size']}p{t.get('scoo_pieces', 0)}{th}"
         return "default" if t["flags"] & L.MFP_TUNE_IS_DEFAULT else f"ss{t['scoo_slice_size']}"
     if fmt in ("hyb", "hyb_atomic"):
         w = t["hyb_width"]
@@ -310,10 +311,14 @@ def measure(matrix: "E.DeviceMatrix", reps: int = 3, flush: bool = True, row_ptr
 
 def tuning_keys(feats) -> list:
     """Lookup keys from the most to the least specific: (floor log2(1 + mean
-    row length), floor log2(rows + 1)), then the mean bin alone, then none."""
+    row length), floor log2(rows + 1), column spread bin), then the first two
+    alone, then the mean bin, then none.  The spread bin comes from feature 58
+    (mean row span over cols): 0 below 0.05 (banded), 1 below 0.3, else 2;
+    it decides how far column sorting can bring a slice's x reads together."""
     m = int(round(feats[3] * 16 * 1e6)) // 1000000
     r = int(round(feats[0] * 32 * 1e6)) // 1000000
-    return [f"{m},{r}", f"{m}", ""]
+    s = 0 if feats[58] < 0.05 else (1 if feats[58] < 0.3 else 2)
+    return [f"{m},{r},{s}", f"{m},{r}", f"{m}", ""]
 
 
 def fit_tuning_rules(records, min_count: int = 3) -> dict:

@@ -32,7 +32,9 @@ def test_tuning_labels():
     assert sel.tuning_label("csr_adaptive", dict(base, nnz_per_wg=1020), marks) == "W/2"
     assert sel.tuning_label("hyb", base, marks) == "rule"
     assert sel.tuning_label("hyb", dict(base, hyb_width=7), marks) == "q75"  # first matching mark
-    assert sel.tuning_label("scoo", dict(base, flags=4, scoo_slice_size=16384), marks) == "cs16384"
+    assert sel.tuning_label("scoo", dict(base, flags=4, scoo_slice_size=16384), marks) == "cs16384p0"
+    assert sel.tuning_label("scoo", dict(base, flags=4, scoo_slice_size=8192, scoo_pieces=2,
+                                         scoo_threads=1024), marks) == "cs8192p2t1024"
     assert sel.tuning_label("scoo", dict(base, flags=2), marks) == "default"
     assert sel.tuning_label("ell", base, marks) == "default"
 
@@ -62,3 +64,20 @@ def test_rules_pick_the_label_with_the_lowest_mean_slowdown_per_key():
     probs[:, sel.FORMAT_ORDER.index("csr_vector")] = 1
     ev = sel.evaluate_with_rules(recs[:2], probs, rules)
     assert ev["mean_regret"] == 0 and ev["within_5pct"] == 1
+
+
+def test_rules_split_by_column_spread():
+    # same mean and row bins: banded matrices (feature 58 small) prefer the
+    # small column-sorted slices, spread-out ones the large slices
+    recs = []
+    for k in range(4):
+        a = _rec(3.5 / 16, 20.2 / 32, [["scoo", "cs4096p0", 1.0, True], ["scoo", "cs16384p0", 1.4, False]])
+        b = _rec(3.5 / 16, 20.2 / 32, [["scoo", "cs4096p0", 2.0, True], ["scoo", "cs16384p0", 1.0, False]])
+        a["features"][58], b["features"][58] = 0.01, 0.8
+        recs += [a, b]
+    rules = sel.fit_tuning_rules(recs)
+    assert rules["scoo"]["3,20,0"][0] == "cs4096p0"
+    assert rules["scoo"]["3,20,2"][0] == "cs16384p0"
+    f = [0.0] * 64
+    f[3], f[0], f[58] = 3.5 / 16, 20.2 / 32, 0.5
+    assert sel.choose_label(rules, "scoo", f, {"cs4096p0", "cs16384p0"}) == "cs16384p0"
