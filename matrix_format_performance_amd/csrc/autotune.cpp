@@ -64,7 +64,7 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
   auto want = [&](int f) { return (format_mask >> f) & 1u; };
   for (int f = 0; f < MFP_FORMAT_COUNT; f++) {
     if (!want(f)) continue;
-    if (lp && f != MFP_CSR_VECTOR && f != MFP_ELL && f != MFP_CSR_ADAPTIVE) continue;
+    if (lp && f != MFP_CSR_VECTOR && f != MFP_ELL && f != MFP_CSR_ADAPTIVE && f != MFP_SCOO) continue;
     mfp_tuning t;
     st = mfp_tuning_default(m, f, &t);
     if (st != MFP_OK) return st;
@@ -94,6 +94,7 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
       push(c, f, tagged);
       const uint32_t acc = 4096u; // 32 KB of double accumulators
       for (uint32_t ss : {256u, 512u, 1024u, 2048u, 4096u}) {
+        if (lp) break; // 16-bit values: the column-sorted form only
         if (v.rows && ss > std::max<uint32_t>(256, v.rows)) break;
         t.scoo_slice_size = ss;
         t.scoo_lanes = std::max<uint32_t>(1, std::min<uint32_t>(32, prev_pow2(acc / ss)));
@@ -115,6 +116,7 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
             tc.scoo_slice_size = R;
             tc.scoo_pieces = P;
             tc.scoo_threads = th;
+            if (lp && R == def.scoo_slice_size && P == 0 && th == def.scoo_threads) continue; // the default, above
             push(c, f, tc);
           }
       }

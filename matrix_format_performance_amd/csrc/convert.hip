@@ -835,7 +835,9 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   st = dmalloc((void **)&out->items, std::max<size_t>(items.size(), 1) * sizeof(uint4));
   if (st == MFP_OK) st = dmalloc((void **)&out->fin, std::max<size_t>(fin.size(), 1) * sizeof(uint4));
   if (st == MFP_OK) st = dmalloc((void **)&out->base, std::max<size_t>(items.size(), 1) * 4);
-  if (st == MFP_OK && slots) st = dmalloc(&out->partial, (uint64_t)slots * R * vs);
+  // partial slots hold the vector type: float for FP32 and 16-bit values
+  const size_t ps = dtype == MFP_F64 ? 8 : 4;
+  if (st == MFP_OK && slots) st = dmalloc(&out->partial, (uint64_t)slots * R * ps);
   if (st == MFP_OK) st = dmalloc(&out->ev, std::max<uint64_t>(nnz, 1) * vs);
   if (st == MFP_OK) st = dmalloc((void **)&out->ec, std::max<uint64_t>(nnz, 1) * 4);
   if (st != MFP_OK) return done(st);
@@ -860,7 +862,7 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
     return done(MFP_ERR_STATE);
   }
   if (nnz) {
-    st = (dtype == MFP_F32 ? cs_scatter_t<float> : cs_scatter_t<double>)(
+    st = (dtype == MFP_F32 ? cs_scatter_t<float> : dtype == MFP_F64 ? cs_scatter_t<double> : cs_scatter_t<uint16_t>)(
         nnz, cur[2], cur[1], lrow, val, out->NI, out->items, out->base, rbits, out->ev, out->ec, s);
     if (st != MFP_OK) return done(st);
   }
@@ -905,7 +907,7 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   }
   out->device_bytes = nnz * (vs + 4) + ((uint64_t)S + 1) * 4 +
                       (uint64_t)out->NI * 20 + (uint64_t)out->NF * 16 + (uint64_t)NL * 4 +
-                      (NL ? ((uint64_t)rows + 31) / 32 * 4 : 0) + (uint64_t)slots * R * vs;
+                      (NL ? ((uint64_t)rows + 31) / 32 * 4 : 0) + (uint64_t)slots * R * ps;
   return done(MFP_OK);
 }
 

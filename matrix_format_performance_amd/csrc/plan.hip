@@ -154,8 +154,9 @@ static int plan_free_arrays(mfp_plan p)
   return MFP_OK;
 }
 
-// Plans over 16-bit values: CSR-vector and ELL with Dot2-compensated FP32
-// accumulation.
+// Plans over 16-bit values: CSR-vector, binned CSR and ELL with
+// Dot2-compensated FP32 accumulation, and column-sorted SCOO with exact
+// products added in double.
 static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipStream_t s);
 
 static uint64_t budget_of(const mfp_tuning &t)
@@ -168,8 +169,11 @@ static uint64_t budget_of(const mfp_tuning &t)
 
 static int matrix_stats(mfp_matrix m, mfp_row_stats *out, hipStream_t s);
 
+static int plan_create_scoo_cs(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipStream_t s);
+
 static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipStream_t s)
 {
+  if (p->format == MFP_SCOO && (t.flags & MFP_TUNE_SCOO_COLSORT)) return plan_create_scoo_cs(p, m, t, s);
   if (p->format == MFP_CSR_VECTOR) {
     if (t.lanes < 2 || t.lanes > 64 || (t.lanes & (t.lanes - 1))) {
       set_error("lanes must be a power of two in [2, 64]");
@@ -183,7 +187,8 @@ static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipS
     return MFP_OK;
   }
   if (p->format != MFP_ELL) {
-    set_error("16-bit values are available in the CSR-vector, CSR-Adaptive (binned) and ELL formats");
+    set_error("16-bit values are available in the CSR-vector, CSR-Adaptive (binned), ELL and column-sorted SCOO "
+              "formats");
     return MFP_ERR_ARG;
   }
   mfp_row_stats rs;
@@ -398,7 +403,11 @@ int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *t)
   scoo_mi355x_geometry(m->dtype, m->rows, &t->scoo_slices, &t->scoo_slice_size, &t->scoo_lanes);
   t->scoo_threads = 512;
   t->budget_bytes = 0;
-  (void)format;
+  if (is_lp(m->dtype) && format == MFP_SCOO) {
+    // 16-bit values have only the column-sorted SCOO
+    t->flags = MFP_TUNE_SCOO_COLSORT;
+    t->scoo_slice_size = 8192;
+  }
   return MFP_OK;
 }
 
@@ -569,6 +578,7 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
       return launch_csr_vector_lp(dt, p->t.lanes, m->rows, m->row_ptr, m->columns, m->values, x, y, s);
     if (p->format == MFP_CSR_ADAPTIVE)
       return launch_lp_binned(dt, p->lpb, m->row_ptr, m->columns, m->values, x, y, s);
+    if (p->format == MFP_SCOO) return launch_scoo_cs(dt, m->rows, p->cs, p->scoo_threads, x, y, s);
     return launch_ell_lp(dt, p->off16, p->lp_block, m->rows, p->ell_w, p->lp_cols, p->lp_base, p->ell_val, x, y, s);
   }
   switch (p->format) {

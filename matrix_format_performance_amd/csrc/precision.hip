@@ -14,6 +14,7 @@
 //   16-bit values makes 4 bytes per stored entry instead of 8.
 #include "kernels.h"
 #include "mfp_internal.h"
+#include "values16.h"
 #include "wave64.h"
 
 #include <hip/hip_bf16.h>
@@ -27,21 +28,6 @@
 
 namespace mfp {
 
-// ---- value loads -------------------------------------------------------------
-__device__ __forceinline__ float lp_to_f32_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ float lp_to_f32_f16(uint16_t h)
-{
-  return __half2float(__ushort_as_half(h));
-}
-
-template <int DT>
-__device__ __forceinline__ float lp_val(uint16_t v)
-{
-  if constexpr (DT == MFP_BF16)
-    return lp_to_f32_bf16(v);
-  else
-    return lp_to_f32_f16(v);
-}
 
 template <typename T>
 __device__ __forceinline__ double widen(const T *p, uint64_t i, int dt)

@@ -15,6 +15,7 @@
 // COO, SCOO and the hybrids accumulate with atomics (order not fixed).
 #include "kernels.h"
 #include "mfp_internal.h"
+#include "values16.h"
 #include "wave64.h"
 
 #include <type_traits>
@@ -630,14 +631,53 @@ __device__ __forceinline__ bool cs_is_long(const uint32_t *__restrict__ mask, ui
   return mask && ((mask[row >> 5] >> (row & 31u)) & 1u);
 }
 
-template <typename T>
+// Stored values of a column-sorted plan: the vector type T itself (DT < 0)
+// or 16-bit BF16 / FP16 (DT = MFP_BF16 / MFP_F16, with T = float).  Every
+// product is formed in double, exactly: a float, BF16 or FP16 value times a
+// float x needs at most 48 significant bits.
+template <typename T, int DT>
+struct cs_values {
+  using V = typename std::conditional<(DT < 0), T, uint16_t>::type;
+  __device__ __forceinline__ static double one(const V *__restrict__ ev, uint32_t i)
+  {
+    if constexpr (DT < 0)
+      return (double)ldg_stream(ev + i);
+    else
+      return (double)lp_val<DT>(ldg_stream(ev + i));
+  }
+  // the four values of 16-byte entry word g
+  __device__ __forceinline__ static void word(const V *__restrict__ ev, uint32_t g, double (&v)[4])
+  {
+    if constexpr (DT >= 0) {
+      const u32x2 h = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(ev) + g);
+      v[0] = (double)lp_val<DT>((uint16_t)(h[0] & 0xFFFFu));
+      v[1] = (double)lp_val<DT>((uint16_t)(h[0] >> 16));
+      v[2] = (double)lp_val<DT>((uint16_t)(h[1] & 0xFFFFu));
+      v[3] = (double)lp_val<DT>((uint16_t)(h[1] >> 16));
+    } else if constexpr (sizeof(T) == 4) {
+      const f32x4 f = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(ev) + g);
+#pragma unroll
+      for (uint32_t j = 0; j < 4; j++) v[j] = (double)f[j];
+    } else {
+      const f64x2 d0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(ev) + 2 * g);
+      const f64x2 d1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(ev) + 2 * g + 1);
+      v[0] = d0[0];
+      v[1] = d0[1];
+      v[2] = d1[0];
+      v[3] = d1[1];
+    }
+  }
+};
+
+template <typename T, int DT>
 __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_reg, uint32_t NL,
                           const uint4 *__restrict__ items, const uint32_t *__restrict__ ibase,
-                          const T *__restrict__ ev, const uint32_t *__restrict__ ec,
+                          const typename cs_values<T, DT>::V *__restrict__ ev, const uint32_t *__restrict__ ec,
                           const T *__restrict__ x, T *__restrict__ y,
                           T *__restrict__ partial, const uint32_t *__restrict__ long_rows,
                           const uint32_t *__restrict__ long_mask)
 {
+  using CV = cs_values<T, DT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double *acc = reinterpret_cast<double *>(smem_raw);
   const uint4 it = items[xcd_block(blockIdx.x, gridDim.x)];
@@ -652,7 +692,7 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
   for (uint32_t i = e0 + tid; i < ea; i += bt) {
     uint32_t c, lr;
     cs_decode(ldg_stream(ec + i), cbase, rbits, c, lr);
-    atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
+    atomicAdd(acc + lr, CV::one(ev, i) * (double)x[c]);
   }
   // body: whole blocks of 256 positions, stored so that 16-byte word l of a
   // block holds positions l, 64 + l, 128 + l, 192 + l; two words per lane in
@@ -661,23 +701,11 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
   uint32_t g = g0 + tid;
   for (; g + bt < g1; g += 2 * bt) {
     u32x4 w[2];
-    T v[2][4];
+    double v[2][4];
 #pragma unroll
     for (uint32_t k = 0; k < 2; k++) {
-      const uint32_t gg = g + k * bt;
-      w[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(ec) + gg);
-      if constexpr (sizeof(T) == 4) {
-        const f32x4 vv = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(ev) + gg);
-#pragma unroll
-        for (uint32_t j = 0; j < 4; j++) v[k][j] = vv[j];
-      } else {
-        const f64x2 v0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(ev) + 2 * gg);
-        const f64x2 v1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(ev) + 2 * gg + 1);
-        v[k][0] = v0[0];
-        v[k][1] = v0[1];
-        v[k][2] = v1[0];
-        v[k][3] = v1[1];
-      }
+      w[k] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(ec) + g + k * bt);
+      CV::word(ev, g + k * bt, v[k]);
     }
     uint32_t c[2][4], lr[2][4];
     T xv[2][4];
@@ -692,23 +720,24 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
 #pragma unroll
     for (uint32_t k = 0; k < 2; k++)
 #pragma unroll
-      for (uint32_t j = 0; j < 4; j++) atomicAdd(acc + lr[k][j], (double)v[k][j] * (double)xv[k][j]);
+      for (uint32_t j = 0; j < 4; j++) atomicAdd(acc + lr[k][j], v[k][j] * (double)xv[k][j]);
   }
   for (; g < g1; g += bt) {
     const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(ec) + g);
+    double v[4];
+    CV::word(ev, g, v);
 #pragma unroll
     for (uint32_t j = 0; j < 4; j++) {
-      const uint32_t i = 4 * g + j;
       uint32_t c, lr;
       cs_decode(w[j], cbase, rbits, c, lr);
-      atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
+      atomicAdd(acc + lr, v[j] * (double)x[c]);
     }
   }
   // tail: entries after the last whole group
   for (uint32_t i = max(ea, g1 << 2) + tid; i < e1; i += bt) {
     uint32_t c, lr;
     cs_decode(ldg_stream(ec + i), cbase, rbits, c, lr);
-    atomicAdd(acc + lr, (double)ldg_stream(ev + i) * (double)x[c]);
+    atomicAdd(acc + lr, CV::one(ev, i) * (double)x[c]);
   }
   __syncthreads();
   if (slot == CS_DIRECT && sl >= S_reg) {
@@ -960,17 +989,18 @@ int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t thre
   return launched();
 }
 
-template <typename T>
+template <typename T, int DT>
 static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
                      hipStream_t s)
 {
+  using V = typename cs_values<T, DT>::V;
   const size_t lds = (size_t)l.R * sizeof(double);
   if (lds > 65536)
-    MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T, DT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-  k_scoo_cs<T><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
-                                                  (const T *)l.ev, l.ec, (const T *)x, (T *)y,
-                                                  (T *)l.partial, l.long_rows, l.long_mask);
+  k_scoo_cs<T, DT><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
+                                              (const V *)l.ev, l.ec, (const T *)x, (T *)y, (T *)l.partial,
+                                              l.long_rows, l.long_mask);
   MFP_CHECK(launched());
   if (l.NF) {
     k_scoo_cs_finish<T><<<l.NF, CS_FIN_ROWS, 0, s>>>(rows, l.R, l.S_reg, l.NL, l.NFR, l.fin, l.long_rows,
@@ -993,8 +1023,14 @@ int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t t
     set_error("SCOO workgroup size must be a multiple of 64 up to 1024");
     return MFP_ERR_ARG;
   }
-  if (dtype == MFP_F32) return scoo_cs_t<float>(rows, l, threads, x, y, s);
-  return scoo_cs_t<double>(rows, l, threads, x, y, s);
+  switch (dtype) {
+  case MFP_F32: return scoo_cs_t<float, -1>(rows, l, threads, x, y, s);
+  case MFP_F64: return scoo_cs_t<double, -1>(rows, l, threads, x, y, s);
+  case MFP_BF16: return scoo_cs_t<float, MFP_BF16>(rows, l, threads, x, y, s);
+  case MFP_F16: return scoo_cs_t<float, MFP_F16>(rows, l, threads, x, y, s);
+  }
+  set_error("column-sorted SCOO: unknown value type");
+  return MFP_ERR_ARG;
 }
 
 } // namespace mfp

@@ -134,3 +134,18 @@ def test_autotune_sweeps_and_times_candidates(gpu):
     y_seq = oracle.spmv_seq(rp, ci, va, x.to_host())
     _, absdot = oracle.spmv_f64(rp, ci, va, x.to_host())
     assert oracle.tolerance_check(y.to_host(), y_seq, absdot, 1e-5) <= 1e-5
+
+
+def test_autotune_16bit_values_offer_column_sorted_scoo_only(gpu):
+    rp, ci, va, cols = case("powerlaw_20k", "float32")
+    announce("autotune bf16 powerlaw_20k")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols).convert("bf16")
+    x = gpu.DeviceArray.from_host(gpu.generate_vector(1, cols, np.float32))
+    y = gpu.DeviceArray(rp.size - 1, np.float32)
+    res = gpu.autotune(A, x, y, reps=1)
+    assert {r["format"] for r in res} == {"csr_vector", "csr_adaptive", "ell", "scoo"}
+    sc = [r for r in res if r["format"] == "scoo"]
+    flag = gpu._lib.MFP_TUNE_SCOO_COLSORT
+    assert sc and all(r["tuning"]["flags"] & flag for r in sc)
+    assert sum(bool(r["tuning"]["flags"] & gpu._lib.MFP_TUNE_IS_DEFAULT) for r in sc) == 1
+    assert all(r["status"] == 0 for r in sc)
