@@ -203,8 +203,9 @@ def cpu_baseline(rp, ci, va, x, nbytes, threads):
 
 
 def precision_study(mfp, timer, flush, reps, cache):
-    """FP32 plain form against BF16 / FP16 value storage (FP32 accumulation,
-    compensated row sums) on the 3D 27-point 128^3 stencil and the power-law
+    """FP32 plain form against BF16 / FP16 value storage (FP32 accumulation
+    with compensated row sums, or exact products added in double in the
+    column-sorted SCOO) on the 3D 27-point 128^3 stencil and the power-law
     matrix: best time per value type, speed-up over FP32, and the error
     against the GPU FP64 oracle of the stored values (arithmetic) and of the
     FP32 values (arithmetic + quantisation)."""
@@ -223,9 +224,11 @@ def precision_study(mfp, timer, flush, reps, cache):
             s_v = 4 if vt == "f32" else 2
             nb = b_alg(rows, rows, nnz, s_v, 4)
             best = None
-            for fmt in ("csr_vector", "csr_adaptive", "ell"):
+            for fmt, kw in (("csr_vector", {}), ("csr_adaptive", {}), ("ell", {}),
+                            ("scoo", {"flags": mfp._lib.MFP_TUNE_SCOO_COLSORT, "scoo_slice_size": 16384,
+                                      "scoo_threads": 1024})):
                 try:
-                    p = mfp.Plan(M, fmt)
+                    p = mfp.Plan(M, fmt, **kw)
                 except mfp.MfpBudgetError:
                     continue
                 time_plan(mfp, p, xd, yd, 1, flush, timer)
