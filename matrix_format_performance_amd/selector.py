@@ -13,12 +13,13 @@ autotuner stays the fallback and the judge of the pick's regret.
 
 Command line (``python -m matrix_format_performance_amd.selector``):
 
-* ``sweep --out S.jsonl --count N [--max-nnz M] [--seed K]``: generates
-  matrices, records features and candidate timings, one JSON line per matrix;
-  rerunning with the same file resumes after the last complete line.
-* ``train --data S.jsonl --out W.npz [--train 1600]``: trains on the first
-  ``--train`` records, fits the tuning rules, saves both (``Selector.load``)
-  and reports regret on the rest.
+* ``sweep --out S.jsonl --count N [--start I] [--max-nnz M] [--min-rows R]
+  [--seed K]``: generates matrices I..N-1, records features and candidate
+  timings, one JSON line per matrix; rerunning with the same file resumes
+  after the last complete line.
+* ``train --data S.jsonl [T.jsonl ...] --out W.npz [--hold-every 5]``: holds
+  out every fifth record, trains on the others, fits the tuning rules, saves
+  both (``Selector.load``) and reports regret on the held-out records.
 """
 from __future__ import annotations
 
@@ -436,7 +437,7 @@ def _done_indices(path: str) -> set:
 
 
 def sweep(out: str, count: int, seed: int = 0, max_nnz: int = 1 << 24, reps: int = 3, start: int = 0,
-          deadline_s: float = None, log=print) -> int:
+          deadline_s: float = None, log=print, min_rows: int = 10_000) -> int:
     """Measure matrices start..count-1 into the JSONL file ``out``, skipping
     those it already holds; returns how many were measured in this call."""
     done = _done_indices(out)
@@ -449,7 +450,7 @@ def sweep(out: str, count: int, seed: int = 0, max_nnz: int = 1 << 24, reps: int
             if deadline_s is not None and time.perf_counter() - t_start > deadline_s:
                 log(f"deadline reached after {measured} matrices")
                 break
-            spec = sample_spec(i, seed, max_nnz)
+            spec = sample_spec(i, seed, max_nnz, min_rows)
             rp, ci, va = E.generate(spec, np.float32)
             cols = {"stencil2d": spec.n * spec.n, "stencil3d": spec.n ** 3}.get(spec.family, rp.size - 1)
             A = E.DeviceMatrix.from_csr(rp, ci, va, cols)
@@ -531,12 +532,14 @@ def main(argv=None) -> int:
     s.add_argument("--start", type=int, default=0)
     s.add_argument("--seed", type=int, default=0)
     s.add_argument("--max-nnz", type=int, default=1 << 24)
+    s.add_argument("--min-rows", type=int, default=10_000)
     s.add_argument("--reps", type=int, default=3)
     s.add_argument("--deadline", type=float, default=None, help="stop after this many seconds")
     t = sub.add_parser("train")
-    t.add_argument("--data", required=True)
+    t.add_argument("--data", required=True, nargs="+", help="sweep JSONL files (records joined in order)")
     t.add_argument("--out", required=True)
-    t.add_argument("--train", type=int, default=1600)
+    t.add_argument("--hold-every", type=int, default=5,
+                   help="hold out every n-th record (by position) for the evaluation; 0 = none")
     t.add_argument("--epochs", type=int, default=300)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=0)
@@ -547,11 +550,13 @@ def main(argv=None) -> int:
     E.init()
     if a.cmd == "sweep":
         n = sweep(a.out, a.count, a.seed, a.max_nnz, a.reps, a.start, a.deadline,
-                  log=lambda m: print(m, flush=True))
+                  log=lambda m: print(m, flush=True), min_rows=a.min_rows)
         print(json.dumps({"measured": n, "total": len(_done_indices(a.out))}))
         return 0
-    recs = load_records(a.data)
-    tr, te = recs[:a.train], recs[a.train:]
+    recs = [r for path in a.data for r in load_records(path)]
+    k = a.hold_every
+    tr = [r for i, r in enumerate(recs) if not (k and i % k == k - 1)]
+    te = [r for i, r in enumerate(recs) if k and i % k == k - 1]
     log = lambda m: print(m, flush=True)  # noqa: E731
     # hyper-parameters on a validation split of the training matrices
     grid = [(tau, ep) for tau in (0.02, 0.05, 0.1, 0.2) for ep in (150, 300, 600)] if a.tune else [(a.tau, a.epochs)]
