@@ -817,7 +817,7 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
     const uint32_t mode = n == 1 ? CS_DIRECT : slots;
     if (n > 1 && sl < S_reg) {
       const uint32_t lim = (uint32_t)std::min<uint64_t>(R, (uint64_t)rows - (uint64_t)sl * R);
-      for (uint32_t r0 = 0; r0 < lim; r0 += CS_FIN_ROWS) fin.push_back(make_uint4(sl, slots, n, r0));
+      for (uint32_t r0 = 0; r0 < lim; r0 += 4 * CS_FIN_ROWS) fin.push_back(make_uint4(sl, slots, n, r0));
     } else if (n > 1) {
       // a long-row slice can have many items: a wave per row adds its slots
       const uint32_t lim = (uint32_t)std::min<uint64_t>(R, (uint64_t)NL - (uint64_t)(sl - S_reg) * R);

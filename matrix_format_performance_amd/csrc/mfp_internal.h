@@ -97,7 +97,7 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_
 // is one workgroup; mode is a partial slot, or CS_DIRECT when the item is its
 // slice's only one and writes the slice's rows of y.  fin[] lists finish
 // blocks {slice, first slot, slots, first row}: the first NFR cover
-// CS_FIN_ROWS rows of an ordinary slice (a thread per row), the rest
+// 4 CS_FIN_ROWS rows of an ordinary slice (a thread per four rows), the rest
 // CS_FIN_LONG rows of a long-row slice (a wave per row, lanes over slots).
 constexpr uint32_t CS_DIRECT = 0xFFFFFFFEu, CS_FIN_ROWS = 256, CS_FIN_LONG = CS_FIN_ROWS / 64;
 struct scoo_cs_layout {

@@ -753,9 +753,45 @@ __global__ void k_scoo_cs(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_
   }
 }
 
-// One workgroup per finish block.  Blocks below NFR: CS_FIN_ROWS rows of an
-// ordinary slice, a thread per row, y[row] = the slice's slots added in slot
-// order (long rows skipped).  The others: CS_FIN_LONG rows of a long-row
+// four consecutive values p[0..3] (16-byte aligned)
+template <typename T>
+__device__ __forceinline__ void load4(const T *__restrict__ p, T (&v)[4])
+{
+  if constexpr (sizeof(T) == 4) {
+    const f32x4 a = *reinterpret_cast<const f32x4 *>(p);
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) v[j] = a[j];
+  } else {
+    const f64x2 a = *reinterpret_cast<const f64x2 *>(p);
+    const f64x2 b = *reinterpret_cast<const f64x2 *>(p + 2);
+    v[0] = a[0];
+    v[1] = a[1];
+    v[2] = b[0];
+    v[3] = b[1];
+  }
+}
+
+// sum_in_order for four adjacent rows at once: d[j] = p[j] + p[stride + j]
+// + ... in slot order, with 16-byte loads (p 16-byte aligned, stride a
+// multiple of 4)
+template <typename T>
+__device__ __forceinline__ void sum4_in_order(const T *__restrict__ p, uint32_t n, uint64_t stride, T (&d)[4])
+{
+  load4(p, d);
+  for (uint32_t k = 1; k < n; k += 4) {
+    T q[4][4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) load4(p + (uint64_t)min(k + i, n - 1) * stride, q[i]);
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++)
+#pragma unroll
+      for (uint32_t j = 0; j < 4; j++) d[j] = (k + i < n) ? d[j] + q[i][j] : d[j];
+  }
+}
+
+// One workgroup per finish block.  Blocks below NFR: 4 CS_FIN_ROWS rows of
+// an ordinary slice, four consecutive rows per thread, y[row] = the slice's
+// slots added in slot order (long rows skipped).  The others: CS_FIN_LONG rows of a long-row
 // slice, a wave per row; lane l adds slots l, l + 64, ... in order and the
 // wave's halving tree adds the lanes, into y[long_rows[(slice - S_reg) R + lr]].
 template <typename T>
@@ -778,11 +814,21 @@ __global__ __launch_bounds__(CS_FIN_ROWS) void k_scoo_cs_finish(uint32_t rows, u
     return;
   }
   const uint32_t lim = min(R, rows - sl * R);
-  const uint32_t lr = f.w + threadIdx.x;
-  if (lr >= lim || threadIdx.x >= CS_FIN_ROWS) return;
-  const uint32_t row = sl * R + lr;
-  if (cs_is_long(long_mask, row)) return;
-  y[row] = sum_in_order(partial + (uint64_t)slot0 * R + lr, n, R);
+  const uint32_t lr0 = f.w + 4 * threadIdx.x;
+  if (lr0 >= lim || threadIdx.x >= CS_FIN_ROWS) return;
+  const T *p = partial + (uint64_t)slot0 * R + lr0;
+  T d[4];
+  if ((R & 3u) == 0 && lr0 + 4 <= lim) {
+    sum4_in_order(p, n, R, d);
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) d[j] = lr0 + j < lim ? sum_in_order(p + j, n, R) : (T)0;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint32_t row = sl * R + lr0 + j;
+    if (lr0 + j < lim && !cs_is_long(long_mask, row)) y[row] = d[j];
+  }
 }
 
 // ---------------------------------------------------------------------------
