@@ -177,6 +177,89 @@ def test_scoo_colsort_wide_columns_cut_at_bands(gpu, dtype, P):
     _tol_check(yd.to_host(), rp, ci, va, x)
 
 
+def _colsort_run(gpu, rp, ci, va, cols, label, **tuning):
+    announce(f"scoo colsort {label} rows={rp.size - 1} nnz={ci.size} tuning={tuning}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, **tuning)
+    x = gpu.generate_vector(9, cols, va.dtype)
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(rp.size - 1, va.dtype)
+    yd.fill(np.nan)
+    plan.spmv(xd, yd)
+    gpu.synchronize()
+    return yd.to_host(), x, plan.info()
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("R", [256, 16384])
+def test_scoo_colsort_empty_matrix_writes_zeros(gpu, dtype, R):
+    rows = 1000
+    rp = np.zeros(rows + 1, np.uint32)
+    y, _, info = _colsort_run(gpu, rp, np.zeros(0, np.uint32), np.zeros(0, np.dtype(dtype)), 700, "empty",
+                              scoo_slice_size=R)
+    assert info["scoo_colsort"] == 1
+    assert np.array_equal(y, np.zeros(rows, np.dtype(dtype)))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("P", [0, 3])
+def test_scoo_colsort_only_the_last_column_band(gpu, dtype, P):
+    # 16384-row slices: bands of 262,144 columns; every entry sits in the
+    # third band, so the first two bands of every slice are empty
+    rng = np.random.default_rng(11)
+    rows, cols = 40000, 600000
+    lens = rng.integers(0, 9, rows)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    ci = np.concatenate([np.sort(rng.choice(np.arange(524288, cols), n, replace=False)) for n in lens]).astype(np.uint32)
+    va = rng.uniform(-1, 1, ci.size).astype(np.dtype(dtype))
+    y, x, info = _colsort_run(gpu, rp, ci, va, cols, "last band", scoo_slice_size=16384, scoo_pieces=P)
+    assert info["scoo_packed"] == 1
+    _tol_check(y, rp, ci, va, x)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("P", [1, 4])
+def test_scoo_colsort_long_rows_fill_several_slices(gpu, dtype, P):
+    # 600 rows of 300 entries among 3,000 rows of 2: with 256-row slices the
+    # long-row threshold is 256, so the 600 rows make three long-row slices
+    rng = np.random.default_rng(12)
+    rows, cols = 3000, 5000
+    lens = np.full(rows, 2)
+    lens[rng.choice(rows, 600, replace=False)] = 300
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    ci = np.concatenate([np.sort(rng.choice(cols, n, replace=False)) for n in lens]).astype(np.uint32)
+    va = rng.uniform(-1, 1, ci.size).astype(np.dtype(dtype))
+    y, x, info = _colsort_run(gpu, rp, ci, va, cols, "long rows", scoo_slice_size=256, scoo_pieces=P)
+    assert info["scoo_long_rows"] == 600
+    assert info["slices_count"] == -(-rows // 256) + 3
+    _tol_check(y, rp, ci, va, x)
+
+
+@pytest.mark.parametrize("vt", ["bf16", "f16"])
+def test_scoo_colsort_16bit_values_in_the_last_band(gpu, vt):
+    rng = np.random.default_rng(13)
+    rows, cols = 20000, 600000
+    lens = rng.integers(0, 9, rows)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    ci = np.concatenate([np.sort(rng.choice(np.arange(524288, cols), n, replace=False)) for n in lens]).astype(np.uint32)
+    va = rng.uniform(-1, 1, ci.size).astype(np.float32)
+    announce(f"scoo colsort {vt} last band")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols).convert(vt)
+    _, _, stored = A.download()
+    vs = ((stored.astype(np.uint32) << 16).view(np.float32) if vt == "bf16"
+          else stored.view(np.float16).astype(np.float32))
+    plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=16384, scoo_pieces=3)
+    x = gpu.generate_vector(9, cols, np.float32)
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(rows, np.float32)
+    yd.fill(np.nan)
+    plan.spmv(xd, yd)
+    gpu.synchronize()
+    y64, ad = oracle.spmv_f64(rp, ci, vs, x)
+    # exact products added in double, partial slots rounded to FP32
+    assert oracle.tolerance_check(yd.to_host(), y64, ad, 1.0) <= 1e-6
+
+
 def test_scoo_colsort_rejects_bad_slices(gpu):
     rp, ci, va, cols = case("stencil2d_64", "float32")
     A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
