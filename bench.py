@@ -73,21 +73,27 @@ def run_batch(mfp, args, timer):
     t0 = time.time()
     members = batch_members(mfp)
     dms = [mfp.DeviceMatrix.from_csr(rp, ci, va) for rp, ci, va in members]
-    # lanes per row for the k > 1 launch: the mean row length rounded up to a power of two
+    # lanes per row for the k > 1 launch: the mean row length rounded up to a
+    # power of two for every row, or each row's own length (binned, lanes 0).
+    # By default both are timed for each k and the faster is reported.
     tot_rows = sum(m[0].size - 1 for m in members)
     tot_nnz = sum(m[1].size for m in members)
-    lanes = args.batch_lanes or int(min(64, max(2, 1 << max(0, int(np.ceil(np.log2(max(1.0, tot_nnz / tot_rows))))))))
-    batch = mfp.Batch(dms, lanes=lanes)
+    mean_lanes = int(min(64, max(2, 1 << max(0, int(np.ceil(np.log2(max(1.0, tot_nnz / tot_rows))))))))
+    modes = {"mean": mean_lanes, "binned": 0}
+    if args.batch_lanes > 0:
+        modes = {f"lanes{args.batch_lanes}": args.batch_lanes}
+    elif args.batch_lanes == 0:
+        modes = {"binned": 0}
+    batches = {name: mfp.Batch(dms, lanes=ln) for name, ln in modes.items()}
     for d in dms:
         d.destroy()
+    batch = next(iter(batches.values()))
     rows, cols, nnz = batch.rows, batch.cols, batch.nnz
     log(f"[bench] batch of {batch.count}: rows={rows} nnz={nnz} (built in {time.time() - t0:.1f}s)")
     del members
-    res = {}
-    for k in (1, 4, 16):
-        X = mfp.DeviceArray.from_host(mfp.generate_vector(2, cols * k, np.float32))
-        Y = mfp.DeviceArray(rows * k, np.float32)
-        g = batch.graph(k, X, Y)
+
+    def time_graph(b, k, X, Y):
+        g = b.graph(k, X, Y)
         for _ in range(args.warmup):
             mfp.flush_cache(1 << 30)
             g.launch()
@@ -99,20 +105,34 @@ def run_batch(mfp, args, timer):
             g.launch()
             timer.stop()
             ms.append(timer.ms())
-        mean_ms = statistics.fmean(ms)
+        g.destroy()
+        return statistics.fmean(ms)
+
+    res = {}
+    for k in (1, 4, 16):
+        X = mfp.DeviceArray.from_host(mfp.generate_vector(2, cols * k, np.float32))
+        Y = mfp.DeviceArray(rows * k, np.float32)
+        # k = 1 is the CSR-Adaptive launch in every batch
+        timed = {name: time_graph(b, k, X, Y) for name, b in (batches.items() if k > 1 else [("adaptive", batch)])}
+        mode = min(timed, key=timed.get)
+        mean_ms = timed[mode]
         nb = nnz * 8 + (rows + 1) * 4 + k * (cols + rows) * 4
         res[k] = {"ms": round(mean_ms, 5), "GB_per_s": round(nb / (mean_ms * 1e-3) / 1e9, 1),
-                  "gflops": round(2 * nnz * k / (mean_ms * 1e-3) / 1e9, 1), "algorithmic_bytes": nb}
+                  "gflops": round(2 * nnz * k / (mean_ms * 1e-3) / 1e9, 1), "algorithmic_bytes": nb,
+                  "lanes": {"adaptive": "CSR-Adaptive", "mean": mean_lanes, "binned": "per row (2..64)"}.get(
+                      mode, args.batch_lanes),
+                  "ms_by_lanes": {n: round(v, 5) for n, v in timed.items()}}
         if k == 1:
             # against the FP64 oracle on the GPU over the assembled matrix
+            batch.spmm(1, X, Y)
             m = mfp.engine.C.c_void_p()
             mfp._lib.call("mfp_batch_matrix", batch.handle, mfp.engine.C.byref(m))
             A = mfp.DeviceMatrix(m.value)
             y64, ad = A.oracle_f64(X)
             res[k]["max_row_err_vs_fp64"] = mfp.residual(Y, y64, ad)["max_row_err"]
             A.handle = None  # owned by the batch
-        log(f"[bench] batch k={k}: {mean_ms * 1e3:.1f} us {res[k]['GB_per_s']} GB/s {res[k]['gflops']} GFLOP/s")
-        g.destroy()
+        log(f"[bench] batch k={k}: {mean_ms * 1e3:.1f} us {res[k]['GB_per_s']} GB/s {res[k]['gflops']} GFLOP/s "
+            f"({mode}; {', '.join(f'{n} {v * 1e3:.1f} us' for n, v in timed.items())})")
         X.free()
         Y.free()
     value = res[1]["GB_per_s"]
@@ -121,7 +141,7 @@ def run_batch(mfp, args, timer):
         "ms_per_step": res[1]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batch512", "description": WORKLOADS["batch512"], "matrices": batch.count,
-                   "spmm_lanes_per_row": lanes,
+                   "spmm_lanes": "per k in batch[k].lanes (mean row length rounded up, or per row)",
                    "rows": rows, "cols": cols, "nnz": nnz, "launch": "one persistent launch, HIP graph replay",
                    "cache": "Infinity Cache evicted before every step"},
         "pct_hbm_roofline": round(100 * value / HBM_PEAK_GBS, 2),
@@ -281,7 +301,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 / FP16 study")
-    ap.add_argument("--batch-lanes", type=int, default=0, help="batch512: lanes per row for k > 1 (0 = auto)")
+    ap.add_argument("--batch-lanes", type=int, default=-1,
+                    help="batch512: lanes per row for k > 1 (-1 = time the mean-based count and per-row lanes, "
+                         "keep the faster; 0 = per row from its length; N = N for every row)")
     ap.add_argument("--values", default=None, choices=["bf16", "f16"],
                     help="store the matrix values in 16 bits (FP32 vectors and accumulation)")
     ap.add_argument("--bandwidth", type=float, default=1.0,

@@ -7,8 +7,9 @@
 // Batch: many matrices of uneven sizes become one block-diagonal CSR in HBM
 // (row pointers offset by the preceding nnz, columns by the preceding
 // columns); X / Y are the matrices' vectors stacked in the same order.  One
-// persistent launch (CSR-Adaptive for k = 1, SpMM otherwise) covers the whole
-// batch, and the launch sequence can be captured once in a HIP graph and
+// persistent launch (CSR-Adaptive for k = 1, SpMM otherwise: one lane count
+// for all rows, or rows binned by length with matched lane counts) covers the
+// whole batch, and the launch sequence can be captured once in a HIP graph and
 // replayed.
 #include "kernels.h"
 #include "mfp_internal.h"
@@ -19,6 +20,16 @@
 #include <vector>
 
 namespace mfp {
+
+// Binned SpMM: six bins, G = 2 << b lanes for rows of at most 2 << b
+// entries (one entry per lane), G = 64 above 32.
+constexpr uint32_t SPMM_BINS = 6;
+inline uint32_t spmm_bin_of(uint32_t len)
+{
+  uint32_t b = 0;
+  while (b + 1 < SPMM_BINS && (2u << b) < len) b++;
+  return b;
+}
 
 // A row of X (KV values, aligned to KV * sizeof(T)) with 16-byte loads where
 // the row is a multiple of 16 bytes.
@@ -43,6 +54,52 @@ __device__ __forceinline__ void load_row(const T *__restrict__ p, T (&out)[KV])
   }
 }
 
+// One row of Y in the CSR-vector order with G lanes (row >= rows: the lanes
+// only take part in the reduction).  Every lane of the wave must call it.
+template <typename T, int G, int KV>
+__device__ __forceinline__ void spmm_row(uint32_t row, uint32_t rows, uint32_t lane, const uint32_t *__restrict__ rp,
+                                         const uint32_t *__restrict__ ci, const T *__restrict__ a,
+                                         const T *__restrict__ X, T *__restrict__ Y)
+{
+  T acc[KV];
+#pragma unroll
+  for (int j = 0; j < KV; j++) acc[j] = 0;
+  if (row < rows) {
+    const uint32_t b = rp[row], e = rp[row + 1];
+    uint32_t i = b + lane;
+    // four entries per lane in flight (matrix and X rows), then fma in entry order
+    for (; i + 3u * G < e; i += 4u * G) {
+      T av[4];
+      const T *xr[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        av[q] = ldg_stream(a + i + q * G);
+        xr[q] = X + (uint64_t)ldg_stream(ci + i + q * G) * KV;
+      }
+      T xv[4][KV];
+#pragma unroll
+      for (int q = 0; q < 4; q++) load_row<T, KV>(xr[q], xv[q]);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int j = 0; j < KV; j++) acc[j] = fma_t(av[q], xv[q][j], acc[j]);
+    }
+    for (; i < e; i += G) {
+      const T av = ldg_stream(a + i);
+      T xv[KV];
+      load_row<T, KV>(X + (uint64_t)ldg_stream(ci + i) * KV, xv);
+#pragma unroll
+      for (int j = 0; j < KV; j++) acc[j] = fma_t(av, xv[j], acc[j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < KV; j++) acc[j] = group_reduce<G>(acc[j]);
+  if (lane == 0 && row < rows) {
+#pragma unroll
+    for (int j = 0; j < KV; j++) Y[(uint64_t)row * KV + j] = acc[j];
+  }
+}
+
 template <typename T, int G, int KV, bool PERSISTENT>
 __global__ __launch_bounds__(256) void k_spmm_vector(uint32_t rows, const uint32_t *__restrict__ rp,
                                                      const uint32_t *__restrict__ ci,
@@ -53,45 +110,54 @@ __global__ __launch_bounds__(256) void k_spmm_vector(uint32_t rows, const uint32
   const uint32_t rows_per_block = 256u / G;
   const uint32_t nblk = (rows + rows_per_block - 1) / rows_per_block;
   for (uint32_t blk = blockIdx.x; blk < nblk; blk += (PERSISTENT ? gridDim.x : nblk)) {
-    const uint32_t row = blk * rows_per_block + threadIdx.x / G;
-    T acc[KV];
-#pragma unroll
-    for (int j = 0; j < KV; j++) acc[j] = 0;
-    if (row < rows) {
-      const uint32_t b = rp[row], e = rp[row + 1];
-      uint32_t i = b + lane;
-      // four entries per lane in flight (matrix and X rows), then fma in entry order
-      for (; i + 3u * G < e; i += 4u * G) {
-        T av[4];
-        const T *xr[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          av[q] = ldg_stream(a + i + q * G);
-          xr[q] = X + (uint64_t)ldg_stream(ci + i + q * G) * KV;
-        }
-        T xv[4][KV];
-#pragma unroll
-        for (int q = 0; q < 4; q++) load_row<T, KV>(xr[q], xv[q]);
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-          for (int j = 0; j < KV; j++) acc[j] = fma_t(av[q], xv[q][j], acc[j]);
-      }
-      for (; i < e; i += G) {
-        const T av = ldg_stream(a + i);
-        T xv[KV];
-        load_row<T, KV>(X + (uint64_t)ldg_stream(ci + i) * KV, xv);
-#pragma unroll
-        for (int j = 0; j < KV; j++) acc[j] = fma_t(av, xv[j], acc[j]);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < KV; j++) acc[j] = group_reduce<G>(acc[j]);
-    if (lane == 0 && row < rows) {
-#pragma unroll
-      for (int j = 0; j < KV; j++) Y[(uint64_t)row * KV + j] = acc[j];
-    }
+    spmm_row<T, G, KV>(blk * rows_per_block + threadIdx.x / G, rows, lane, rp, ci, a, X, Y);
     if (!PERSISTENT) break;
+  }
+}
+
+// Rows binned by length: bin b (G = 2 << b lanes per row) holds the rows
+// list[off[b] .. off[b + 1]); a tile is 256 / G of them, one workgroup.  The
+// persistent grid walks the tiles of all bins (tile_off[b] = first tile of
+// bin b), so each row is summed in the CSR-vector order with its bin's G.
+// Binning fills the lanes of short rows but gives up the matrix's streaming
+// order (a wave's rows are no longer adjacent): it pays when the X rows are
+// wide (k = 16 in the 512-matrix batch), not at k = 4.
+struct spmm_bins_args {
+  uint32_t off[SPMM_BINS + 1];
+  uint32_t tile_off[SPMM_BINS + 1];
+};
+
+template <typename T, int G, int KV>
+__device__ __forceinline__ void spmm_bin_tile(uint32_t t, uint32_t n, const uint32_t *__restrict__ list,
+                                              uint32_t rows, const uint32_t *__restrict__ rp,
+                                              const uint32_t *__restrict__ ci, const T *__restrict__ a,
+                                              const T *__restrict__ X, T *__restrict__ Y)
+{
+  const uint32_t idx = t * (256u / G) + threadIdx.x / G;
+  const uint32_t row = idx < n ? list[idx] : rows; // rows: no row, the lanes still reduce
+  spmm_row<T, G, KV>(row, rows, threadIdx.x % G, rp, ci, a, X, Y);
+}
+
+template <typename T, int KV>
+__global__ __launch_bounds__(256) void k_spmm_binned(spmm_bins_args bins, const uint32_t *__restrict__ list,
+                                                     uint32_t rows, const uint32_t *__restrict__ rp,
+                                                     const uint32_t *__restrict__ ci, const T *__restrict__ a,
+                                                     const T *__restrict__ X, T *__restrict__ Y)
+{
+  const uint32_t total = bins.tile_off[SPMM_BINS];
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    uint32_t b = 0;
+    while (b + 1 < SPMM_BINS && t >= bins.tile_off[b + 1]) b++;
+    const uint32_t tt = t - bins.tile_off[b], n = bins.off[b + 1] - bins.off[b];
+    const uint32_t *l = list + bins.off[b];
+    switch (b) {
+    case 0: spmm_bin_tile<T, 2, KV>(tt, n, l, rows, rp, ci, a, X, Y); break;
+    case 1: spmm_bin_tile<T, 4, KV>(tt, n, l, rows, rp, ci, a, X, Y); break;
+    case 2: spmm_bin_tile<T, 8, KV>(tt, n, l, rows, rp, ci, a, X, Y); break;
+    case 3: spmm_bin_tile<T, 16, KV>(tt, n, l, rows, rp, ci, a, X, Y); break;
+    case 4: spmm_bin_tile<T, 32, KV>(tt, n, l, rows, rp, ci, a, X, Y); break;
+    default: spmm_bin_tile<T, 64, KV>(tt, n, l, rows, rp, ci, a, X, Y); break;
+    }
   }
 }
 
@@ -144,6 +210,61 @@ int launch_spmm_vector(int dtype, uint32_t k, uint32_t lanes, uint32_t rows, con
                     : spmm_k<double, false>(k, lanes, rows, rp, ci, (const double *)a, (const double *)X, (double *)Y, grid, s);
 }
 
+template <typename T>
+static int spmm_binned_t(uint32_t k, const spmm_bins_args &bins, const uint32_t *list, uint32_t rows,
+                         const uint32_t *rp, const uint32_t *ci, const T *a, const T *X, T *Y, hipStream_t s)
+{
+  const uint32_t tiles = bins.tile_off[SPMM_BINS];
+  if (tiles == 0) return MFP_OK;
+  const uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)std::max(1, props().cu_count) * 8u);
+  switch (k) {
+  case 1: k_spmm_binned<T, 1><<<grid, 256, 0, s>>>(bins, list, rows, rp, ci, a, X, Y); break;
+  case 2: k_spmm_binned<T, 2><<<grid, 256, 0, s>>>(bins, list, rows, rp, ci, a, X, Y); break;
+  case 4: k_spmm_binned<T, 4><<<grid, 256, 0, s>>>(bins, list, rows, rp, ci, a, X, Y); break;
+  case 8: k_spmm_binned<T, 8><<<grid, 256, 0, s>>>(bins, list, rows, rp, ci, a, X, Y); break;
+  case 16: k_spmm_binned<T, 16><<<grid, 256, 0, s>>>(bins, list, rows, rp, ci, a, X, Y); break;
+  default: set_error("k must be 1, 2, 4, 8 or 16"); return MFP_ERR_ARG;
+  }
+  return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
+}
+
+static int launch_spmm_binned(int dtype, uint32_t k, const spmm_bins_args &bins, const uint32_t *list,
+                              uint32_t rows, const uint32_t *rp, const uint32_t *ci, const void *a,
+                              const void *X, void *Y, hipStream_t s)
+{
+  if ((k * dtype_size(dtype)) % 16 == 0 && ((uintptr_t)X % 16) != 0) {
+    set_error("X must be 16-byte aligned when its rows are a multiple of 16 bytes");
+    return MFP_ERR_ARG;
+  }
+  if (dtype == MFP_F32)
+    return spmm_binned_t<float>(k, bins, list, rows, rp, ci, (const float *)a, (const float *)X, (float *)Y, s);
+  return spmm_binned_t<double>(k, bins, list, rows, rp, ci, (const double *)a, (const double *)X, (double *)Y, s);
+}
+
+// Row lists per bin (row order kept inside a bin) from the row pointers.
+static int spmm_bins_build(uint32_t rows, const uint32_t *rp_dev, spmm_bins_args *bins, uint32_t **list_out,
+                           hipStream_t s)
+{
+  std::vector<uint32_t> rp((size_t)rows + 1);
+  MFP_HIP(hipMemcpyAsync(rp.data(), rp_dev, rp.size() * 4, hipMemcpyDeviceToHost, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> cnt(SPMM_BINS, 0);
+  for (uint32_t r = 0; r < rows; r++) cnt[spmm_bin_of(rp[r + 1] - rp[r])]++;
+  bins->off[0] = 0;
+  bins->tile_off[0] = 0;
+  for (uint32_t b = 0; b < SPMM_BINS; b++) {
+    bins->off[b + 1] = bins->off[b] + cnt[b];
+    const uint32_t per_tile = 256u / (2u << b);
+    bins->tile_off[b + 1] = bins->tile_off[b] + (cnt[b] + per_tile - 1) / per_tile;
+  }
+  std::vector<uint32_t> list((size_t)std::max<uint32_t>(rows, 1)), pos(bins->off, bins->off + SPMM_BINS);
+  for (uint32_t r = 0; r < rows; r++) list[pos[spmm_bin_of(rp[r + 1] - rp[r])]++] = r;
+  MFP_CHECK(dmalloc((void **)list_out, list.size() * 4));
+  MFP_HIP(hipMemcpyAsync(*list_out, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  return MFP_OK;
+}
+
 // ---- batch assembly ---------------------------------------------------------------
 __global__ void k_offset_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t n,
                              uint32_t add)
@@ -172,7 +293,9 @@ struct mfp_batch_s {
   mfp_matrix big = nullptr; // block-diagonal CSR
   std::vector<uint32_t> row_off, col_off;
   mfp_plan adaptive = nullptr; // k = 1
-  uint32_t lanes = 8;          // k > 1
+  uint32_t lanes = 8;          // k > 1; 0: lanes per row from its length (binned)
+  spmm_bins_args bins{};       // lanes == 0
+  uint32_t *bin_list = nullptr;
 };
 
 struct mfp_graph_s {
@@ -200,6 +323,9 @@ static int batch_launch(mfp_batch b, uint32_t k, const void *X, void *Y, hipStre
     g.persistent = true;
     return launch_csr_adaptive(m->dtype, g, m->values, X, Y, s);
   }
+  if (b->lanes == 0)
+    return launch_spmm_binned(m->dtype, k, b->bins, b->bin_list, m->rows, m->row_ptr, m->columns, m->values, X,
+                              Y, s);
   return launch_spmm_vector(m->dtype, k, b->lanes, m->rows, m->row_ptr, m->columns, m->values, X, Y,
                             true, s);
 }
@@ -221,7 +347,7 @@ int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint3
                      const mfp_tuning *adaptive_tuning, mfp_stream sv)
 {
   MFP_ARG(out && ms && count > 0, "null pointer or empty batch");
-  MFP_ARG(lanes >= 2 && lanes <= 64 && (lanes & (lanes - 1)) == 0, "lanes");
+  MFP_ARG(lanes == 0 || (lanes >= 2 && lanes <= 64 && (lanes & (lanes - 1)) == 0), "lanes");
   MFP_CHECK(ensure_init());
   hipStream_t s = as_stream(sv);
   const int dt = ms[0]->dtype;
@@ -273,6 +399,7 @@ int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint3
   }
   if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
   if (st == MFP_OK) st = mfp_plan_create(&b->adaptive, m, MFP_CSR_ADAPTIVE, adaptive_tuning, sv);
+  if (st == MFP_OK && lanes == 0) st = spmm_bins_build(m->rows, m->row_ptr, &b->bins, &b->bin_list, s);
   if (st != MFP_OK) {
     mfp_batch_destroy(b);
     return st;
@@ -352,6 +479,7 @@ int mfp_batch_destroy(mfp_batch b)
   if (!b) return MFP_OK;
   mfp_plan_destroy(b->adaptive);
   mfp_matrix_destroy(b->big);
+  dfree(b->bin_list);
   delete b;
   return MFP_OK;
 }

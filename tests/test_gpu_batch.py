@@ -86,6 +86,43 @@ def test_batch_persistent_launch(gpu, k, dtype):
     assert np.array_equal(Y, ref)
 
 
+def _binned_lanes(rp):
+    """Lanes per row of a batch created with lanes = 0: 2 << b for rows of at
+    most 2 << b entries, b <= 5."""
+    lens = np.diff(rp.astype(np.int64))
+    b = np.zeros(lens.size, np.int64)
+    for q in range(5):
+        b += lens > (2 << q)
+    return 2 << b
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("k", [2, 4, 16])
+def test_batch_binned_lanes_per_row(gpu, k, dtype):
+    announce(f"batch binned k={k} dtype={dtype}")
+    members = _batch_members(gpu, 24, dtype, seed=20 + k)
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    batch = gpu.Batch(dms, lanes=0, long_row=512, chunk=256)
+    for d in dms:
+        d.destroy()
+    rp, ci, va = batch.matrix_arrays()
+    X = np.random.default_rng(5).uniform(-1, 1, (batch.cols, k)).astype(dtype)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray(batch.rows * k, np.dtype(dtype))
+    Yd.fill(np.nan)
+    batch.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Y = Yd.to_host().reshape(-1, k)
+    G = _binned_lanes(rp)
+    assert set(np.unique(G)) >= {2, 64}  # short rows and long rows both present
+    ref = np.empty_like(Y)
+    for g in np.unique(G):
+        sel = G == g
+        ref[sel] = _columns_ref(rp, ci, va, X, int(g))[sel]
+    # each column in the CSR-vector order with the row's own lane count
+    assert np.array_equal(Y, ref)
+
+
 def test_batch_graph_replays_match_eager(gpu):
     announce("batch graph replay")
     members = _batch_members(gpu, 16, "float32", seed=7)
