@@ -205,15 +205,19 @@ typedef struct mfp_graph_s *mfp_graph;
 /* lanes: CSR-vector lanes per row for k > 1 (2..64), or 0 to give each row
  * the next power of two of its length, from 2 to 64 lanes; each column is
  * then summed in the CSR-vector order with its row's lane count. */
+/* k1_tuning: the plan for k = 1 over the assembled matrix: CSR-Adaptive
+ * (NULL = its defaults), or the column-sorted SCOO when its flags hold
+ * MFP_TUNE_SCOO_COLSORT (two kernels, not bit-exact to a row order). */
 int mfp_batch_create(mfp_batch *out, const mfp_matrix *matrices, uint32_t count, uint32_t lanes,
-                     const mfp_tuning *adaptive_tuning /* NULL = defaults */, mfp_stream s);
+                     const mfp_tuning *k1_tuning, mfp_stream s);
 /* offsets arrays get count + 1 entries (may be NULL) */
 int mfp_batch_info(mfp_batch b, uint32_t *rows, uint32_t *cols, uint64_t *nnz, uint32_t *row_offsets,
                    uint32_t *col_offsets);
 /* the assembled block-diagonal matrix (owned by the batch) */
 int mfp_batch_matrix(mfp_batch b, mfp_matrix *out);
-/* one persistent launch: CSR-Adaptive for k = 1, CSR-vector SpMM for k > 1
- * (binned by row length when the batch was created with lanes = 0) */
+/* one persistent launch: CSR-Adaptive (or the column-sorted SCOO plan) for
+ * k = 1, CSR-vector SpMM for k > 1 (binned by row length when the batch was
+ * created with lanes = 0) */
 int mfp_batch_spmm(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_stream s);
 /* capture that launch sequence once in a HIP graph; replay with mfp_graph_launch */
 int mfp_batch_graph_create(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_graph *out);

@@ -7,7 +7,8 @@
 // Batch: many matrices of uneven sizes become one block-diagonal CSR in HBM
 // (row pointers offset by the preceding nnz, columns by the preceding
 // columns); X / Y are the matrices' vectors stacked in the same order.  One
-// persistent launch (CSR-Adaptive for k = 1, SpMM otherwise: one lane count
+// persistent launch (CSR-Adaptive for k = 1, or the column-sorted SCOO's two
+// kernels when asked; SpMM otherwise: one lane count
 // for all rows, or rows binned by length with matched lane counts) covers the
 // whole batch, and the launch sequence can be captured once in a HIP graph and
 // replayed.
@@ -292,7 +293,7 @@ struct mfp_batch_s {
   int dtype = MFP_F32;
   mfp_matrix big = nullptr; // block-diagonal CSR
   std::vector<uint32_t> row_off, col_off;
-  mfp_plan adaptive = nullptr; // k = 1
+  mfp_plan k1 = nullptr;       // k = 1: CSR-Adaptive, or column-sorted SCOO
   uint32_t lanes = 8;          // k > 1; 0: lanes per row from its length (binned)
   spmm_bins_args bins{};       // lanes == 0
   uint32_t *bin_list = nullptr;
@@ -307,7 +308,8 @@ static int batch_launch(mfp_batch b, uint32_t k, const void *X, void *Y, hipStre
 {
   const mfp_matrix m = b->big;
   if (k == 1) {
-    const mfp_plan p = b->adaptive;
+    const mfp_plan p = b->k1;
+    if (p->format == MFP_SCOO) return launch_scoo_cs(m->dtype, m->rows, p->cs, p->scoo_threads, X, Y, s);
     adaptive_args g;
     g.rows = m->rows;
     g.rp = m->row_ptr;
@@ -344,7 +346,7 @@ int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream sv)
 }
 
 int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint32_t lanes,
-                     const mfp_tuning *adaptive_tuning, mfp_stream sv)
+                     const mfp_tuning *k1_tuning, mfp_stream sv)
 {
   MFP_ARG(out && ms && count > 0, "null pointer or empty batch");
   MFP_ARG(lanes == 0 || (lanes >= 2 && lanes <= 64 && (lanes & (lanes - 1)) == 0), "lanes");
@@ -398,7 +400,10 @@ int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint3
     e0 += a->nnz;
   }
   if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
-  if (st == MFP_OK) st = mfp_plan_create(&b->adaptive, m, MFP_CSR_ADAPTIVE, adaptive_tuning, sv);
+  if (st == MFP_OK) {
+    const int f = k1_tuning && (k1_tuning->flags & MFP_TUNE_SCOO_COLSORT) ? MFP_SCOO : MFP_CSR_ADAPTIVE;
+    st = mfp_plan_create(&b->k1, m, f, k1_tuning, sv);
+  }
   if (st == MFP_OK && lanes == 0) st = spmm_bins_build(m->rows, m->row_ptr, &b->bins, &b->bin_list, s);
   if (st != MFP_OK) {
     mfp_batch_destroy(b);
@@ -477,7 +482,7 @@ int mfp_graph_destroy(mfp_graph g)
 int mfp_batch_destroy(mfp_batch b)
 {
   if (!b) return MFP_OK;
-  mfp_plan_destroy(b->adaptive);
+  mfp_plan_destroy(b->k1);
   mfp_matrix_destroy(b->big);
   dfree(b->bin_list);
   delete b;

@@ -333,14 +333,16 @@ class Graph:
 
 class Batch:
     """Many matrices assembled into one block-diagonal CSR, multiplied by one
-    persistent launch (CSR-Adaptive for k = 1, CSR-vector SpMM for k > 1)."""
+    persistent launch (CSR-Adaptive for k = 1, or the column-sorted SCOO when
+    the tuning's flags hold MFP_TUNE_SCOO_COLSORT; CSR-vector SpMM for k > 1,
+    lanes = 0: lanes per row from its length)."""
 
-    def __init__(self, matrices, lanes: int = 8, stream=None, **adaptive_tuning):
+    def __init__(self, matrices, lanes: int = 8, stream=None, **k1_tuning):
         arr = (C.c_void_p * len(matrices))(*[m.handle for m in matrices])
         t = None
-        if adaptive_tuning:
+        if k1_tuning:
             t = default_tuning(matrices[0], "csr_adaptive")
-            for k, v in adaptive_tuning.items():
+            for k, v in k1_tuning.items():
                 setattr(t, k, v)
         h = C.c_void_p()
         L.call("mfp_batch_create", C.byref(h), arr, len(matrices), lanes, C.byref(t) if t else None, stream)

@@ -186,3 +186,31 @@ def test_autotune_16bit_values_offer_column_sorted_scoo_only(gpu):
     assert sc and all(r["tuning"]["flags"] & flag for r in sc)
     assert sum(bool(r["tuning"]["flags"] & gpu._lib.MFP_TUNE_IS_DEFAULT) for r in sc) == 1
     assert all(r["status"] == 0 for r in sc)
+
+
+def test_batch_column_sorted_k1(gpu):
+    announce("batch column-sorted k=1")
+    members = _batch_members(gpu, 16, "float32", seed=9)
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    batch = gpu.Batch(dms, lanes=16, flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=4096, scoo_pieces=2)
+    for d in dms:
+        d.destroy()
+    rp, ci, va = batch.matrix_arrays()
+    Xd = gpu.DeviceArray(batch.cols, np.float32)
+    Yg = gpu.DeviceArray(batch.rows, np.float32)
+    Ye = gpu.DeviceArray(batch.rows, np.float32)
+    g = batch.graph(1, Xd, Yg)
+    for rep in range(3):
+        x = np.random.default_rng(40 + rep).uniform(-1, 1, batch.cols).astype(np.float32)
+        gpu.engine.L.call("mfp_memcpy_h2d", Xd.ptr, x.ctypes.data, x.nbytes)
+        Yg.fill(np.nan)
+        g.launch()
+        gpu.synchronize()
+        batch.spmm(1, Xd, Ye)  # eager work between replays
+        gpu.synchronize()
+        _, absdot = oracle.spmv_f64(rp, ci, va, x)
+        y_seq = oracle.spmv_seq(rp, ci, va, x)
+        for y in (Yg.to_host(), Ye.to_host()):
+            assert not np.isnan(y).any()
+            assert oracle.tolerance_check(y, y_seq, absdot, 1e-5) <= 1e-5
+    g.destroy()
