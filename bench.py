@@ -79,20 +79,23 @@ def run_batch(mfp, args, timer):
     tot_rows = sum(m[0].size - 1 for m in members)
     tot_nnz = sum(m[1].size for m in members)
     mean_lanes = int(min(64, max(2, 1 << max(0, int(np.ceil(np.log2(max(1.0, tot_nnz / tot_rows))))))))
-    modes = {"mean": mean_lanes, "binned": 0}
-    if args.batch_lanes > 0:
+    modes = {"mean": mean_lanes, "binned": 0, "colsort": 1}
+    if args.batch_lanes > 1:
         modes = {f"lanes{args.batch_lanes}": args.batch_lanes}
     elif args.batch_lanes == 0:
         modes = {"binned": 0}
+    elif args.batch_lanes == 1:
+        modes = {"colsort": 1}
     # k = 1: CSR-Adaptive (the first batch) or the column-sorted SCOO (the last)
     cs = {"flags": mfp._lib.MFP_TUNE_SCOO_COLSORT, "scoo_slice_size": 16384, "scoo_pieces": 2,
           "scoo_threads": 1024}
     names = list(modes)
-    batches = {n: mfp.Batch(dms, lanes=modes[n], **(cs if n == names[-1] and len(names) > 1 else {}))
-               for n in names}
+    # the first batch keeps CSR-Adaptive for k = 1, the second (or an extra
+    # one) gets the column-sorted SCOO for k = 1
+    batches = {n: mfp.Batch(dms, lanes=modes[n], **(cs if i == 1 else {})) for i, n in enumerate(names)}
     if len(names) == 1:
-        batches["colsort"] = mfp.Batch(dms, lanes=modes[names[0]], **cs)
-    k1 = {"adaptive": batches[names[0]], "colsort": batches[names[-1]] if len(names) > 1 else batches["colsort"]}
+        batches["k1_colsort"] = mfp.Batch(dms, lanes=modes[names[0]], **cs)
+    k1 = {"adaptive": batches[names[0]], "colsort": batches[names[1] if len(names) > 1 else "k1_colsort"]}
     for d in dms:
         d.destroy()
     batch = batches[names[0]]
@@ -127,7 +130,8 @@ def run_batch(mfp, args, timer):
         nb = nnz * 8 + (rows + 1) * 4 + k * (cols + rows) * 4
         res[k] = {"ms": round(mean_ms, 5), "GB_per_s": round(nb / (mean_ms * 1e-3) / 1e9, 1),
                   "gflops": round(2 * nnz * k / (mean_ms * 1e-3) / 1e9, 1), "algorithmic_bytes": nb,
-                  "plan": {"adaptive": "persistent CSR-Adaptive", "colsort": "column-sorted SCOO, 16384-row slices",
+                  "plan": {"adaptive": "persistent CSR-Adaptive",
+                           "colsort": "column-sorted SCOO" + (", 16384-row slices" if k == 1 else " SpMM"),
                            "mean": f"CSR-vector SpMM, {mean_lanes} lanes", "binned": "CSR-vector SpMM, lanes per row (2..64)"
                            }.get(mode, f"CSR-vector SpMM, {args.batch_lanes} lanes"),
                   "ms_by_plan": {n: round(v, 5) for n, v in timed.items()}}
@@ -312,8 +316,9 @@ def main():
     ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 / FP16 study")
     ap.add_argument("--batch-lanes", type=int, default=-1,
-                    help="batch512: lanes per row for k > 1 (-1 = time the mean-based count and per-row lanes, "
-                         "keep the faster; 0 = per row from its length; N = N for every row)")
+                    help="batch512, k > 1: -1 = time the mean-based lane count, per-row lanes and the "
+                         "column-sorted SpMM and keep the fastest; 0 = per-row lanes; 1 = column-sorted SpMM; "
+                         "N = N lanes for every row")
     ap.add_argument("--values", default=None, choices=["bf16", "f16"],
                     help="store the matrix values in 16 bits (FP32 vectors and accumulation)")
     ap.add_argument("--bandwidth", type=float, default=1.0,

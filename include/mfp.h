@@ -190,21 +190,26 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
 int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream s);
 /* Y = A X for k right-hand sides (k in {1,2,4,8,16}); X is cols x k and Y
  * rows x k, row-major; X is 16-byte aligned when k values make a multiple
- * of 16 bytes.  k > 1 runs on CSR-vector plans (f32 / f64); each
- * column is summed in the CSR-vector order.  No reference counterpart (the
- * reference multiplies one vector). */
+ * of 16 bytes.  k > 1 runs on CSR-vector plans (f32 / f64; each column is
+ * summed in the CSR-vector order) and on column-sorted SCOO plans (f32 /
+ * f64, while scoo_slice_size * k doubles fit in the 160 KB LDS: k <= 4 for
+ * 4096-row slices, 16 for 1024; exact products added in double per slice
+ * row and column, partial slots added in slot order).  No reference
+ * counterpart (the reference multiplies one vector). */
 int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream s);
 
-/* ---- batches: many matrices in one persistent launch ----------------------
+/* ---- batches: many matrices in one launch ----------------------------------
  * No reference counterpart.  The matrices (same value type, f32 / f64) are
  * assembled on the device into one block-diagonal CSR; X and Y are the
  * matrices' vectors stacked in order (row i of matrix j at row_offsets[j] + i
  * of Y, column offsets likewise for X), k columns each, row-major. */
 typedef struct mfp_batch_s *mfp_batch;
 typedef struct mfp_graph_s *mfp_graph;
-/* lanes: CSR-vector lanes per row for k > 1 (2..64), or 0 to give each row
- * the next power of two of its length, from 2 to 64 lanes; each column is
- * then summed in the CSR-vector order with its row's lane count. */
+/* lanes, for k > 1: CSR-vector lanes per row (2..64); 0 to give each row the
+ * next power of two of its length, from 2 to 64 lanes (each column is then
+ * summed in the CSR-vector order with its row's lane count); 1 for the
+ * column-sorted SCOO SpMM over the assembled matrix (one plan per k, built by
+ * the first mfp_batch_spmm or mfp_batch_graph_create with that k). */
 /* k1_tuning: the plan for k = 1 over the assembled matrix: CSR-Adaptive
  * (NULL = its defaults), or the column-sorted SCOO when its flags hold
  * MFP_TUNE_SCOO_COLSORT (two kernels, not bit-exact to a row order). */
@@ -215,9 +220,8 @@ int mfp_batch_info(mfp_batch b, uint32_t *rows, uint32_t *cols, uint64_t *nnz, u
                    uint32_t *col_offsets);
 /* the assembled block-diagonal matrix (owned by the batch) */
 int mfp_batch_matrix(mfp_batch b, mfp_matrix *out);
-/* one persistent launch: CSR-Adaptive (or the column-sorted SCOO plan) for
- * k = 1, CSR-vector SpMM for k > 1 (binned by row length when the batch was
- * created with lanes = 0) */
+/* one launch: CSR-Adaptive (or the column-sorted SCOO plan) for k = 1; for
+ * k > 1 the SpMM the batch's lanes select */
 int mfp_batch_spmm(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_stream s);
 /* capture that launch sequence once in a HIP graph; replay with mfp_graph_launch */
 int mfp_batch_graph_create(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_graph *out);

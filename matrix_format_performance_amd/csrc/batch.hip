@@ -7,11 +7,11 @@
 // Batch: many matrices of uneven sizes become one block-diagonal CSR in HBM
 // (row pointers offset by the preceding nnz, columns by the preceding
 // columns); X / Y are the matrices' vectors stacked in the same order.  One
-// persistent launch (CSR-Adaptive for k = 1, or the column-sorted SCOO's two
-// kernels when asked; SpMM otherwise: one lane count
-// for all rows, or rows binned by length with matched lane counts) covers the
-// whole batch, and the launch sequence can be captured once in a HIP graph and
-// replayed.
+// launch (CSR-Adaptive for k = 1, or the column-sorted SCOO's two kernels
+// when asked; SpMM otherwise: CSR-vector with one lane count for all rows or
+// rows binned by length with matched lane counts, or the column-sorted SCOO
+// SpMM) covers the whole batch, and the launch sequence can be captured once
+// in a HIP graph and replayed.
 #include "kernels.h"
 #include "mfp_internal.h"
 #include "wave64.h"
@@ -30,29 +30,6 @@ inline uint32_t spmm_bin_of(uint32_t len)
   uint32_t b = 0;
   while (b + 1 < SPMM_BINS && (2u << b) < len) b++;
   return b;
-}
-
-// A row of X (KV values, aligned to KV * sizeof(T)) with 16-byte loads where
-// the row is a multiple of 16 bytes.
-template <typename T, int KV>
-__device__ __forceinline__ void load_row(const T *__restrict__ p, T (&out)[KV])
-{
-  if constexpr ((KV * sizeof(T)) % 16 == 0) {
-    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-    constexpr int N = KV * sizeof(T) / 16;
-    constexpr int PER = 16 / sizeof(T);
-#pragma unroll
-    for (int v = 0; v < N; v++) {
-      const u4 w = *reinterpret_cast<const u4 *>(p + v * PER);
-      T tmp[PER];
-      __builtin_memcpy(tmp, &w, 16);
-#pragma unroll
-      for (int j = 0; j < PER; j++) out[v * PER + j] = tmp[j];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < KV; j++) out[j] = p[j];
-  }
 }
 
 // One row of Y in the CSR-vector order with G lanes (row >= rows: the lanes
@@ -117,9 +94,10 @@ __global__ __launch_bounds__(256) void k_spmm_vector(uint32_t rows, const uint32
 }
 
 // Rows binned by length: bin b (G = 2 << b lanes per row) holds the rows
-// list[off[b] .. off[b + 1]); a tile is 256 / G of them, one workgroup.  The
-// persistent grid walks the tiles of all bins (tile_off[b] = first tile of
-// bin b), so each row is summed in the CSR-vector order with its bin's G.
+// list[off[b] .. off[b + 1]); a tile is 256 / G of them, one workgroup
+// (tile_off[b] = first tile of bin b; the grid-stride loop also serves a
+// smaller grid), so each row is summed in the CSR-vector order with its
+// bin's G.
 // Binning fills the lanes of short rows but gives up the matrix's streaming
 // order (a wave's rows are no longer adjacent): it pays when the X rows are
 // wide (k = 16 in the 512-matrix batch), not at k = 4.
@@ -217,7 +195,7 @@ static int spmm_binned_t(uint32_t k, const spmm_bins_args &bins, const uint32_t 
 {
   const uint32_t tiles = bins.tile_off[SPMM_BINS];
   if (tiles == 0) return MFP_OK;
-  const uint32_t grid = std::min<uint32_t>(tiles, (uint32_t)std::max(1, props().cu_count) * 8u);
+  const uint32_t grid = tiles; // one tile per workgroup (the loop below then runs once)
   switch (k) {
   case 1: k_spmm_binned<T, 1><<<grid, 256, 0, s>>>(bins, list, rows, rp, ci, a, X, Y); break;
   case 2: k_spmm_binned<T, 2><<<grid, 256, 0, s>>>(bins, list, rows, rp, ci, a, X, Y); break;
@@ -297,7 +275,36 @@ struct mfp_batch_s {
   uint32_t lanes = 8;          // k > 1; 0: lanes per row from its length (binned)
   spmm_bins_args bins{};       // lanes == 0
   uint32_t *bin_list = nullptr;
+  mfp_plan cs[5] = {};         // lanes == 1: column-sorted SCOO plan for k = 2 << (i - 1)
 };
+
+static uint32_t log2_k(uint32_t k)
+{
+  uint32_t i = 0;
+  while ((1u << i) < k) i++;
+  return i;
+}
+
+// lanes == 1: the column-sorted plan for k columns, built on first use (never
+// inside a graph capture): the largest slice (up to 16384 rows) whose R k
+// double accumulators fit in LDS, about two items per band, 1024 threads.
+static int batch_cs_plan(mfp_batch b, uint32_t k, mfp_stream sv)
+{
+  const uint32_t i = log2_k(k);
+  if (i == 0 || i > 4 || (1u << i) != k) {
+    set_error("k must be 1, 2, 4, 8 or 16");
+    return MFP_ERR_ARG;
+  }
+  if (b->cs[i]) return MFP_OK;
+  mfp_tuning t;
+  MFP_CHECK(mfp_tuning_default(b->big, MFP_SCOO, &t));
+  t.flags = MFP_TUNE_SCOO_COLSORT;
+  t.scoo_slice_size = 16384;
+  while ((uint64_t)t.scoo_slice_size * k * 8 > 163840) t.scoo_slice_size /= 2;
+  t.scoo_pieces = 2;
+  t.scoo_threads = 1024;
+  return mfp_plan_create(&b->cs[i], b->big, MFP_SCOO, &t, sv);
+}
 
 struct mfp_graph_s {
   hipGraph_t graph = nullptr;
@@ -328,8 +335,16 @@ static int batch_launch(mfp_batch b, uint32_t k, const void *X, void *Y, hipStre
   if (b->lanes == 0)
     return launch_spmm_binned(m->dtype, k, b->bins, b->bin_list, m->rows, m->row_ptr, m->columns, m->values, X,
                               Y, s);
+  if (b->lanes == 1) {
+    const uint32_t i = log2_k(k);
+    if (i > 4 || !b->cs[i]) {
+      set_error("column-sorted batch SpMM: plan for this k missing");
+      return MFP_ERR_STATE;
+    }
+    return launch_scoo_cs_spmm(m->dtype, k, m->rows, b->cs[i]->cs, b->cs[i]->scoo_threads, X, Y, s);
+  }
   return launch_spmm_vector(m->dtype, k, b->lanes, m->rows, m->row_ptr, m->columns, m->values, X, Y,
-                            true, s);
+                            false, s);
 }
 
 extern "C" {
@@ -338,7 +353,9 @@ int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream sv)
 {
   MFP_ARG(p && X && Y, "null pointer");
   if (k == 1) return mfp_plan_spmv(p, X, Y, sv);
-  MFP_ARG(p->format == MFP_CSR_VECTOR, "SpMM with k > 1 runs on CSR-vector plans");
+  if (p->format == MFP_SCOO && p->colsort)
+    return launch_scoo_cs_spmm(p->dtype, k, p->m->rows, p->cs, p->scoo_threads, X, Y, as_stream(sv));
+  MFP_ARG(p->format == MFP_CSR_VECTOR, "SpMM with k > 1 runs on CSR-vector and column-sorted SCOO plans");
   MFP_ARG(p->dtype == MFP_F32 || p->dtype == MFP_F64, "SpMM values must be f32 or f64");
   const mfp_matrix m = p->m;
   return launch_spmm_vector(p->dtype, k, p->t.lanes, m->rows, m->row_ptr, m->columns, m->values, X, Y,
@@ -349,7 +366,7 @@ int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint3
                      const mfp_tuning *k1_tuning, mfp_stream sv)
 {
   MFP_ARG(out && ms && count > 0, "null pointer or empty batch");
-  MFP_ARG(lanes == 0 || (lanes >= 2 && lanes <= 64 && (lanes & (lanes - 1)) == 0), "lanes");
+  MFP_ARG(lanes <= 1 || (lanes <= 64 && (lanes & (lanes - 1)) == 0), "lanes");
   MFP_CHECK(ensure_init());
   hipStream_t s = as_stream(sv);
   const int dt = ms[0]->dtype;
@@ -435,6 +452,7 @@ int mfp_batch_matrix(mfp_batch b, mfp_matrix *out)
 int mfp_batch_spmm(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_stream sv)
 {
   MFP_ARG(b && X && Y, "null pointer");
+  if (b->lanes == 1 && k > 1) MFP_CHECK(batch_cs_plan(b, k, sv));
   return batch_launch(b, k, X, Y, as_stream(sv));
 }
 
@@ -442,6 +460,7 @@ int mfp_batch_graph_create(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_
 {
   MFP_ARG(b && X && Y && out, "null pointer");
   MFP_CHECK(ensure_init());
+  if (b->lanes == 1 && k > 1) MFP_CHECK(batch_cs_plan(b, k, nullptr)); // before the capture
   hipStream_t cs = nullptr;
   MFP_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   auto *g = new mfp_graph_s;
@@ -483,6 +502,7 @@ int mfp_batch_destroy(mfp_batch b)
 {
   if (!b) return MFP_OK;
   mfp_plan_destroy(b->k1);
+  for (mfp_plan p : b->cs) mfp_plan_destroy(p);
   mfp_matrix_destroy(b->big);
   dfree(b->bin_list);
   delete b;

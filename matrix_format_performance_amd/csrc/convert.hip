@@ -835,9 +835,14 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   st = dmalloc((void **)&out->items, std::max<size_t>(items.size(), 1) * sizeof(uint4));
   if (st == MFP_OK) st = dmalloc((void **)&out->fin, std::max<size_t>(fin.size(), 1) * sizeof(uint4));
   if (st == MFP_OK) st = dmalloc((void **)&out->base, std::max<size_t>(items.size(), 1) * 4);
-  // partial slots hold the vector type: float for FP32 and 16-bit values
+  // partial slots hold the vector type: float for FP32 and 16-bit values;
+  // FP32 / FP64 plans also serve SpMM with k columns while R k doubles fit
+  // in LDS, so their slots hold kmax columns
   const size_t ps = dtype == MFP_F64 ? 8 : 4;
-  if (st == MFP_OK && slots) st = dmalloc(&out->partial, (uint64_t)slots * R * ps);
+  out->kmax = 1;
+  if (dtype == MFP_F32 || dtype == MFP_F64)
+    while (out->kmax < 16 && (uint64_t)R * out->kmax * 2 * 8 <= 163840) out->kmax *= 2;
+  if (st == MFP_OK && slots) st = dmalloc(&out->partial, (uint64_t)slots * R * out->kmax * ps);
   if (st == MFP_OK) st = dmalloc(&out->ev, std::max<uint64_t>(nnz, 1) * vs);
   if (st == MFP_OK) st = dmalloc((void **)&out->ec, std::max<uint64_t>(nnz, 1) * 4);
   if (st != MFP_OK) return done(st);
@@ -907,7 +912,7 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   }
   out->device_bytes = nnz * (vs + 4) + ((uint64_t)S + 1) * 4 +
                       (uint64_t)out->NI * 20 + (uint64_t)out->NF * 16 + (uint64_t)NL * 4 +
-                      (NL ? ((uint64_t)rows + 31) / 32 * 4 : 0) + (uint64_t)slots * R * ps;
+                      (NL ? ((uint64_t)rows + 31) / 32 * 4 : 0) + (uint64_t)slots * R * out->kmax * ps;
   return done(MFP_OK);
 }
 

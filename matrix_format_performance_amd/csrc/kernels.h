@@ -44,6 +44,8 @@ int launch_hyb_atomic(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, 
                       const void *x, void *y, hipStream_t s);
 int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,
                    hipStream_t s);
+int launch_scoo_cs_spmm(int dtype, uint32_t k, uint32_t rows, const scoo_cs_layout &l, uint32_t threads,
+                        const void *X, void *Y, hipStream_t s);
 int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t threads,
                 const uint32_t *cr, const uint32_t *cc, const void *cv, const uint32_t *sp,
                 const void *x, void *y, hipStream_t s);

@@ -103,13 +103,14 @@ constexpr uint32_t CS_DIRECT = 0xFFFFFFFEu, CS_FIN_ROWS = 256, CS_FIN_LONG = CS_
 struct scoo_cs_layout {
   uint32_t R = 0, rbits = 0, S_reg = 0, S_long = 0, S = 0, NL = 0;
   uint32_t NI = 0, NF = 0, NFR = 0, slots = 0;
+  uint32_t kmax = 1;             // widest SpMM (columns of X) the partial slots hold
   uint32_t *sp = nullptr;        // S + 1 entry offsets of the slices
   uint32_t *long_rows = nullptr; // NL
   uint32_t *long_mask = nullptr; // ceil(rows / 32) bit words, when NL > 0
   uint4 *items = nullptr;        // NI
   uint4 *fin = nullptr;          // NF
   uint32_t *base = nullptr;      // NI first column of each item
-  void *partial = nullptr;       // slots * R
+  void *partial = nullptr;       // slots * R * kmax
   void *ev = nullptr;            // values in slice-major, column-sorted order
   uint32_t *ec = nullptr;        // (column - item base) << rbits | local row
   uint64_t device_bytes = 0;

@@ -832,6 +832,112 @@ __global__ __launch_bounds__(CS_FIN_ROWS) void k_scoo_cs_finish(uint32_t rows, u
 }
 
 // ---------------------------------------------------------------------------
+// Column-sorted SCOO SpMM, Y = A X for KV columns (X cols x KV, Y rows x KV,
+// row-major): the SpMV kernel above with KV accumulators per slice row
+// (acc[column][row], doubles, R KV 8 bytes of LDS: lanes adding one column
+// for random rows then spread over the banks, where acc[row][column] would
+// put them KV doubles apart) and one X row (KV values, 16-byte loads) per
+// entry.  Partial slots hold R KV values, row-major like Y.
+// ---------------------------------------------------------------------------
+template <typename T, int KV>
+__global__ void k_scoo_cs_k(uint32_t rows, uint32_t R, uint32_t rbits, uint32_t S_reg, uint32_t NL,
+                            const uint4 *__restrict__ items, const uint32_t *__restrict__ ibase,
+                            const T *__restrict__ ev, const uint32_t *__restrict__ ec, const T *__restrict__ X,
+                            T *__restrict__ Y, T *__restrict__ partial, const uint32_t *__restrict__ long_rows,
+                            const uint32_t *__restrict__ long_mask)
+{
+  using CV = cs_values<T, -1>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *acc = reinterpret_cast<double *>(smem_raw);
+  const uint4 it = items[xcd_block(blockIdx.x, gridDim.x)];
+  const uint32_t sl = it.x, e0 = it.y, e1 = it.z, slot = it.w;
+  const uint32_t lim = sl < S_reg ? min(R, rows - sl * R) : min(R, NL - (sl - S_reg) * R);
+  const uint32_t bt = blockDim.x, tid = threadIdx.x;
+  for (uint32_t i = tid; i < R * KV; i += bt) acc[i] = 0.0;
+  __syncthreads();
+  const uint32_t cbase = ibase[xcd_block(blockIdx.x, gridDim.x)];
+  auto add_row = [&](uint32_t c, uint32_t lr, double v) {
+    T xr[KV];
+    load_row<T, KV>(X + (uint64_t)c * KV, xr);
+#pragma unroll
+    for (int j = 0; j < KV; j++) atomicAdd(acc + j * R + lr, v * (double)xr[j]);
+  };
+  const uint32_t ea = (uint32_t)min<uint64_t>(e1, ((uint64_t)e0 + 255u) & ~(uint64_t)255);
+  for (uint32_t i = e0 + tid; i < ea; i += bt) {
+    uint32_t c, lr;
+    cs_decode(ldg_stream(ec + i), cbase, rbits, c, lr);
+    add_row(c, lr, CV::one(ev, i));
+  }
+  // body: one 16-byte entry word (four entries) per lane per step
+  const uint32_t g0 = ea >> 2, g1 = max(g0, (e1 & ~255u) >> 2);
+  for (uint32_t g = g0 + tid; g < g1; g += bt) {
+    const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(ec) + g);
+    double v[4];
+    CV::word(ev, g, v);
+    uint32_t c[4], lr[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) cs_decode(w[j], cbase, rbits, c[j], lr[j]);
+    T xr[4][KV];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) load_row<T, KV>(X + (uint64_t)c[j] * KV, xr[j]);
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < KV; q++) atomicAdd(acc + q * R + lr[j], v[j] * (double)xr[j][q]);
+  }
+  for (uint32_t i = max(ea, g1 << 2) + tid; i < e1; i += bt) {
+    uint32_t c, lr;
+    cs_decode(ldg_stream(ec + i), cbase, rbits, c, lr);
+    add_row(c, lr, CV::one(ev, i));
+  }
+  __syncthreads();
+  if (slot == CS_DIRECT && sl >= S_reg) {
+    const uint32_t *lrows = long_rows + (uint64_t)(sl - S_reg) * R;
+    for (uint32_t i = tid; i < lim * KV; i += bt)
+      Y[(uint64_t)lrows[i / KV] * KV + i % KV] = (T)acc[(i % KV) * R + i / KV];
+  } else if (slot == CS_DIRECT) {
+    const uint64_t r0 = (uint64_t)sl * R;
+    for (uint32_t i = tid; i < lim * KV; i += bt)
+      if (!cs_is_long(long_mask, (uint32_t)(r0 + i / KV))) Y[r0 * KV + i] = (T)acc[(i % KV) * R + i / KV];
+  } else {
+    T *o = partial + (uint64_t)slot * R * KV;
+    for (uint32_t i = tid; i < lim * KV; i += bt) o[i] = (T)acc[(i % KV) * R + i / KV];
+  }
+}
+
+// The SpMM finish: as k_scoo_cs_finish, over (row, column) pairs; each
+// pair's slots are added in slot order.
+template <typename T>
+__global__ __launch_bounds__(CS_FIN_ROWS) void k_scoo_cs_finish_k(uint32_t rows, uint32_t R, uint32_t KV,
+                                                                  uint32_t S_reg, uint32_t NL, uint32_t NFR,
+                                                                  const uint4 *__restrict__ fin,
+                                                                  const uint32_t *__restrict__ long_rows,
+                                                                  const uint32_t *__restrict__ long_mask,
+                                                                  const T *__restrict__ partial,
+                                                                  T *__restrict__ Y)
+{
+  const uint4 f = fin[blockIdx.x];
+  const uint32_t sl = f.x, slot0 = f.y, n = f.z;
+  const uint64_t stride = (uint64_t)R * KV;
+  const T *p = partial + (uint64_t)slot0 * stride;
+  if (blockIdx.x >= NFR) {
+    const uint32_t lim = min(R, NL - (sl - S_reg) * R);
+    for (uint32_t i = threadIdx.x; i < CS_FIN_LONG * KV; i += CS_FIN_ROWS) {
+      const uint32_t lr = f.w + i / KV, j = i % KV;
+      if (lr < lim) Y[(uint64_t)long_rows[(uint64_t)(sl - S_reg) * R + lr] * KV + j] = sum_in_order(p + (uint64_t)lr * KV + j, n, stride);
+    }
+    return;
+  }
+  const uint32_t lim = min(R, rows - sl * R);
+  for (uint32_t i = threadIdx.x; i < 4 * CS_FIN_ROWS * KV; i += CS_FIN_ROWS) {
+    const uint32_t lr = f.w + i / KV, j = i % KV;
+    const uint32_t row = sl * R + lr;
+    if (lr < lim && !cs_is_long(long_mask, row))
+      Y[(uint64_t)row * KV + j] = sum_in_order(p + (uint64_t)lr * KV + j, n, stride);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static int launched() { return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP); }
@@ -1054,6 +1160,57 @@ static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, c
     MFP_CHECK(launched());
   }
   return MFP_OK;
+}
+
+template <typename T, int KV>
+static int scoo_cs_k_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *X, void *Y,
+                       hipStream_t s)
+{
+  const size_t lds = (size_t)l.R * KV * sizeof(double);
+  if (lds > 65536)
+    MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs_k<T, KV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+  k_scoo_cs_k<T, KV><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
+                                                (const T *)l.ev, l.ec, (const T *)X, (T *)Y, (T *)l.partial,
+                                                l.long_rows, l.long_mask);
+  MFP_CHECK(launched());
+  if (l.NF) {
+    k_scoo_cs_finish_k<T><<<l.NF, CS_FIN_ROWS, 0, s>>>(rows, l.R, KV, l.S_reg, l.NL, l.NFR, l.fin, l.long_rows,
+                                                        l.long_mask, (const T *)l.partial, (T *)Y);
+    MFP_CHECK(launched());
+  }
+  return MFP_OK;
+}
+
+int launch_scoo_cs_spmm(int dtype, uint32_t k, uint32_t rows, const scoo_cs_layout &l, uint32_t threads,
+                        const void *X, void *Y, hipStream_t s)
+{
+  if (dtype != MFP_F32 && dtype != MFP_F64) {
+    set_error("column-sorted SpMM values must be f32 or f64");
+    return MFP_ERR_ARG;
+  }
+  if (k > l.kmax) {
+    set_error("column-sorted SpMM: slice_size * k doubles exceed the 160 KB LDS (use smaller slices)");
+    return MFP_ERR_ARG;
+  }
+  if ((k * dtype_size(dtype)) % 16 == 0 && ((uintptr_t)X % 16) != 0) {
+    set_error("X must be 16-byte aligned when its rows are a multiple of 16 bytes");
+    return MFP_ERR_ARG;
+  }
+  if (threads == 0 || threads > 1024 || threads % 64) {
+    set_error("SCOO workgroup size must be a multiple of 64 up to 1024");
+    return MFP_ERR_ARG;
+  }
+  if (rows == 0 || l.NI == 0) return MFP_OK;
+  const bool f32 = dtype == MFP_F32;
+  switch (k) {
+  case 2: return f32 ? scoo_cs_k_t<float, 2>(rows, l, threads, X, Y, s) : scoo_cs_k_t<double, 2>(rows, l, threads, X, Y, s);
+  case 4: return f32 ? scoo_cs_k_t<float, 4>(rows, l, threads, X, Y, s) : scoo_cs_k_t<double, 4>(rows, l, threads, X, Y, s);
+  case 8: return f32 ? scoo_cs_k_t<float, 8>(rows, l, threads, X, Y, s) : scoo_cs_k_t<double, 8>(rows, l, threads, X, Y, s);
+  case 16: return f32 ? scoo_cs_k_t<float, 16>(rows, l, threads, X, Y, s) : scoo_cs_k_t<double, 16>(rows, l, threads, X, Y, s);
+  }
+  set_error("k must be 1, 2, 4, 8 or 16");
+  return MFP_ERR_ARG;
 }
 
 int launch_scoo_cs(int dtype, uint32_t rows, const scoo_cs_layout &l, uint32_t threads, const void *x, void *y,

@@ -97,6 +97,29 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
   return xcd < rem ? xcd * (q + 1) + idx : rem * (q + 1) + (xcd - rem) * q + idx;
 }
 
+// A row of X (KV values, aligned to KV * sizeof(T)) with 16-byte loads where
+// the row is a multiple of 16 bytes.
+template <typename T, int KV>
+__device__ __forceinline__ void load_row(const T *__restrict__ p, T (&out)[KV])
+{
+  if constexpr ((KV * sizeof(T)) % 16 == 0) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    constexpr int N = KV * sizeof(T) / 16;
+    constexpr int PER = 16 / sizeof(T);
+#pragma unroll
+    for (int v = 0; v < N; v++) {
+      const u4 w = *reinterpret_cast<const u4 *>(p + v * PER);
+      T tmp[PER];
+      __builtin_memcpy(tmp, &w, 16);
+#pragma unroll
+      for (int j = 0; j < PER; j++) out[v * PER + j] = tmp[j];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < KV; j++) out[j] = p[j];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T ldg_stream(const T *p)
 {

@@ -333,9 +333,10 @@ class Graph:
 
 class Batch:
     """Many matrices assembled into one block-diagonal CSR, multiplied by one
-    persistent launch (CSR-Adaptive for k = 1, or the column-sorted SCOO when
-    the tuning's flags hold MFP_TUNE_SCOO_COLSORT; CSR-vector SpMM for k > 1,
-    lanes = 0: lanes per row from its length)."""
+    launch (CSR-Adaptive for k = 1, or the column-sorted SCOO when the
+    tuning's flags hold MFP_TUNE_SCOO_COLSORT; for k > 1 CSR-vector SpMM with
+    ``lanes`` lanes per row, lanes = 0: lanes per row from its length, lanes =
+    1: the column-sorted SCOO SpMM)."""
 
     def __init__(self, matrices, lanes: int = 8, stream=None, **k1_tuning):
         arr = (C.c_void_p * len(matrices))(*[m.handle for m in matrices])

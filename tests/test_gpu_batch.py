@@ -214,3 +214,70 @@ def test_batch_column_sorted_k1(gpu):
             assert not np.isnan(y).any()
             assert oracle.tolerance_check(y, y_seq, absdot, 1e-5) <= 1e-5
     g.destroy()
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("k", [2, 4, 8, 16])
+@pytest.mark.parametrize("name,P", [("stencil3d_20", 0), ("powerlaw_20k", 3), ("edge_rows", 1)])
+def test_spmm_column_sorted_within_tolerance(gpu, name, P, k, dtype):
+    rp, ci, va, cols = case(name, dtype)
+    announce(f"spmm column-sorted {name} k={k} dtype={dtype} P={P}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=1024, scoo_pieces=P)
+    X = np.random.default_rng(k).uniform(-1, 1, (cols, k)).astype(dtype)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray((rp.size - 1) * k, np.dtype(dtype))
+    Yd.fill(np.nan)
+    plan.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Y = Yd.to_host().reshape(-1, k)
+    assert not np.isnan(Y).any()
+    rtol = 1e-5 if dtype == "float32" else 1e-12
+    for j in range(k):
+        x = np.ascontiguousarray(X[:, j])
+        y64, absdot = oracle.spmv_f64(rp, ci, va, x)
+        assert oracle.tolerance_check(Y[:, j], oracle.spmv_seq(rp, ci, va, x), absdot, rtol) <= rtol
+        assert oracle.tolerance_check(Y[:, j], y64, absdot, rtol) <= rtol
+
+
+def test_spmm_column_sorted_refuses_a_slice_too_large_for_k(gpu):
+    rp, ci, va, cols = case("stencil3d_20", "float32")
+    announce("spmm column-sorted refusal")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=4096)
+    X = gpu.DeviceArray(cols * 8, np.float32)
+    Y = gpu.DeviceArray((rp.size - 1) * 8, np.float32)
+    with pytest.raises(gpu.MfpError):
+        plan.spmm(8, X, Y)  # 4096 x 8 doubles = 256 KB of LDS
+    plan.spmm(4, X, Y)      # 128 KB fits
+    gpu.synchronize()
+
+
+@pytest.mark.parametrize("k", [4, 16])
+def test_batch_column_sorted_spmm(gpu, k):
+    announce(f"batch column-sorted spmm k={k}")
+    members = _batch_members(gpu, 16, "float32", seed=30 + k)
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    batch = gpu.Batch(dms, lanes=1)
+    for d in dms:
+        d.destroy()
+    rp, ci, va = batch.matrix_arrays()
+    Xd = gpu.DeviceArray(batch.cols * k, np.float32)
+    Yg = gpu.DeviceArray(batch.rows * k, np.float32)
+    Ye = gpu.DeviceArray(batch.rows * k, np.float32)
+    g = batch.graph(k, Xd, Yg)  # builds the column-sorted plan for k before the capture
+    for rep in range(2):
+        X = np.random.default_rng(50 + rep).uniform(-1, 1, (batch.cols, k)).astype(np.float32)
+        gpu.engine.L.call("mfp_memcpy_h2d", Xd.ptr, X.ctypes.data, X.nbytes)
+        Yg.fill(np.nan)
+        g.launch()
+        gpu.synchronize()
+        batch.spmm(k, Xd, Ye)
+        gpu.synchronize()
+        for Y in (Yg.to_host().reshape(-1, k), Ye.to_host().reshape(-1, k)):
+            assert not np.isnan(Y).any()
+            for j in range(k):
+                x = np.ascontiguousarray(X[:, j])
+                _, absdot = oracle.spmv_f64(rp, ci, va, x)
+                assert oracle.tolerance_check(Y[:, j], oracle.spmv_seq(rp, ci, va, x), absdot, 1e-5) <= 1e-5
+    g.destroy()
