@@ -275,3 +275,24 @@ def test_bench_app_usage():
         pytest.skip("mfp_bench not built")
     r = subprocess.run([app], capture_output=True, text=True, timeout=30)
     assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_bench_refuses_several_gpus_before_touching_one():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""  # no JSON line for an unsupported run
+
+
+def test_bench_algorithmic_bytes_closed_form():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    # 2D 5-point 256^2: 65,536 rows, 326,656 nnz, FP32
+    rows, nnz = 65536, 5 * 65536 - 4 * 256
+    assert nnz == 326656
+    assert bench.b_alg(rows, rows, nnz, 4, 4) == nnz * 8 + (rows + 1) * 4 + rows * 4 + rows * 4 == 3399684
+    # 16-bit values still count 4-byte columns
+    assert bench.b_alg(rows, rows, nnz, 2, 4) == 3399684 - 2 * nnz
