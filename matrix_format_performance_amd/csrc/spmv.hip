@@ -807,28 +807,44 @@ __global__ __launch_bounds__(CS_FIN_ROWS) void k_scoo_cs_finish(uint32_t rows, u
   if (blockIdx.x >= NFR) {
     const uint32_t lr = f.w + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
     if (lr >= min(R, NL - (sl - S_reg) * R)) return; // uniform across the wave
-    T d = (T)0;
-    for (uint32_t k = lane; k < n; k += 64) d = d + partial[(uint64_t)(slot0 + k) * R + lr];
+    // the destination row and up to 8 of the lane's slots are loaded together
+    const uint32_t dst = long_rows[(uint64_t)(sl - S_reg) * R + lr];
+    const uint32_t cnt = lane < n ? (n - lane + 63u) >> 6 : 0u;
+    T d = cnt ? sum_in_order(partial + (uint64_t)(slot0 + lane) * R + lr, cnt, (uint64_t)64 * R) : (T)0;
     d = group_reduce<64>(d);
-    if (lane == 0) y[long_rows[(uint64_t)(sl - S_reg) * R + lr]] = d;
+    if (lane == 0) y[dst] = d;
     return;
   }
   const uint32_t lim = min(R, rows - sl * R);
   const uint32_t lr0 = f.w + 4 * threadIdx.x;
   if (lr0 >= lim || threadIdx.x >= CS_FIN_ROWS) return;
   const T *p = partial + (uint64_t)slot0 * R + lr0;
+  const uint32_t row0 = sl * R + lr0;
   T d[4];
+  uint32_t lm = 0; // bit j: row0 + j is a long row (its y comes from a long-row slice)
   if ((R & 3u) == 0 && lr0 + 4 <= lim) {
+    // row0 is a multiple of 4, so its four mask bits share one word
+    if (long_mask) lm = (long_mask[row0 >> 5] >> (row0 & 31u)) & 15u;
     sum4_in_order(p, n, R, d);
   } else {
 #pragma unroll
-    for (uint32_t j = 0; j < 4; j++) d[j] = lr0 + j < lim ? sum_in_order(p + j, n, R) : (T)0;
+    for (uint32_t j = 0; j < 4; j++) {
+      d[j] = lr0 + j < lim ? sum_in_order(p + j, n, R) : (T)0;
+      lm |= (lr0 + j < lim && cs_is_long(long_mask, row0 + j)) ? 1u << j : 0u;
+    }
+  }
+  if (lm == 0 && lr0 + 4 <= lim && (reinterpret_cast<uintptr_t>(y + row0) & 15u) == 0) {
+    if constexpr (sizeof(T) == 4) {
+      *reinterpret_cast<f32x4 *>(y + row0) = f32x4{d[0], d[1], d[2], d[3]};
+    } else {
+      reinterpret_cast<f64x2 *>(y + row0)[0] = f64x2{d[0], d[1]};
+      reinterpret_cast<f64x2 *>(y + row0)[1] = f64x2{d[2], d[3]};
+    }
+    return;
   }
 #pragma unroll
-  for (uint32_t j = 0; j < 4; j++) {
-    const uint32_t row = sl * R + lr0 + j;
-    if (lr0 + j < lim && !cs_is_long(long_mask, row)) y[row] = d[j];
-  }
+  for (uint32_t j = 0; j < 4; j++)
+    if (lr0 + j < lim && !((lm >> j) & 1u)) y[row0 + j] = d[j];
 }
 
 // ---------------------------------------------------------------------------
