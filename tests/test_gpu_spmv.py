@@ -260,6 +260,41 @@ def test_scoo_colsort_16bit_values_in_the_last_band(gpu, vt):
     assert oracle.tolerance_check(yd.to_host(), y64, ad, 1.0) <= 1e-6
 
 
+class _Offset:
+    """A device pointer one element into a DeviceArray (not 16-byte aligned)."""
+
+    def __init__(self, arr, elements):
+        self.ptr = arr.ptr + elements * arr.dtype.itemsize
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("R,P", [(256, 4), (16384, 2)])
+def test_scoo_colsort_unaligned_y(gpu, dtype, R, P):
+    # the finish stores four rows with one 16-byte write when y + row is
+    # 16-byte aligned; a y one element in must take the per-row stores and
+    # leave the elements on either side alone
+    rng = np.random.default_rng(14)
+    rows, cols = 40000, 30000
+    lens = rng.integers(0, 12, rows)
+    lens[rng.choice(rows, 40, replace=False)] = 2000  # long rows: ordinary slices skip them
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    ci = np.concatenate([np.sort(rng.choice(cols, n, replace=False)) for n in lens]).astype(np.uint32)
+    va = rng.uniform(-1, 1, ci.size).astype(np.dtype(dtype))
+    announce(f"scoo colsort unaligned y dtype={dtype} R={R} P={P}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=R, scoo_pieces=P)
+    assert plan.info()["scoo_long_rows"] > 0
+    x = gpu.generate_vector(15, cols, np.dtype(dtype))
+    xd = gpu.DeviceArray.from_host(x)
+    big = gpu.DeviceArray(rows + 2, np.dtype(dtype))
+    big.fill(np.nan)
+    plan.spmv(xd, _Offset(big, 1))
+    gpu.synchronize()
+    out = big.to_host()
+    assert np.isnan(out[0]) and np.isnan(out[-1])
+    _tol_check(out[1:-1], rp, ci, va, x)
+
+
 def test_scoo_colsort_rejects_bad_slices(gpu):
     rp, ci, va, cols = case("stencil2d_64", "float32")
     A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
