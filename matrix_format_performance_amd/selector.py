@@ -9,7 +9,7 @@ every format, per matrix), and it infers in FP32 or in BF16 on the matrix
 cores.  The pick is the most probable format; its tuning comes from rules
 fitted on the same timings (per bin of mean row length and row count, the
 tuning with the lowest mean slowdown within the format).  The exhaustive
-autotuner stays the fallback and the judge of the pick's regret.
+autotuner stays the fallback and the yardstick for the pick's regret.
 
 Command line (``python -m matrix_format_performance_amd.selector``):
 
