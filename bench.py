@@ -2,10 +2,10 @@
 """SpMV benchmark on one MI355X.
 
 One step = one SpMV y = A x over the whole synthetic matrix, inputs resident
-in HBM.  Before every step (warm-up and timed) a 1 GiB write evicts the
+in HBM.  Before every step (warm-up and timed) a 1 GiB read evicts the
 256 MB Infinity Cache, outside the event pair, so each timed SpMV reads the
-matrix from HBM.  The step is timed with a hipEvent pair on the stream the
-kernels run on.
+matrix from HBM (a read leaves no dirty lines for the SpMV to write back).
+The step is timed with a hipEvent pair on the stream the kernels run on.
 
 value = algorithmic bytes / mean step time, in GB/s, where
   B_alg = nnz*(s_v + 4) + (rows + 1)*4 + cols*s_x + rows*s_y
@@ -14,6 +14,14 @@ count as overhead, not credit).  The plan measured is the fastest autotuner
 candidate (every format and its swept tunings, each timed the same way as
 the steps), reported in config.format / config.tuning, with the best
 candidate of every format in the "formats" table.
+
+roofline: the same flushed steps again, with an event after each kernel of
+the SpMV (mfp_plan_kernel_times); the dominant kernel is the one with the
+longest mean time.  achieved = its algorithmic bytes / that mean, where a
+lone kernel is credited with B_alg and, in a two-kernel SpMV (a main kernel
+then one that adds partial slots or chunk sums into y), the dominant kernel
+with B_alg - rows*s_y (the matrix and x reads).  traffic = its HBM bytes
+per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/traffic_*.json).
 
 cpu_baseline: the oracle's restatement of the reference CPU path
 (cpu_matrix_multiplier.cpp), timed on this host on the same matrix.
@@ -184,6 +192,17 @@ def b_alg(rows, cols, nnz, s_v, s_x):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_names(fmt, info):
+    """The kernels one SpMV of an FP32/FP64 plan launches, in order."""
+    if fmt == "scoo":
+        return ["k_scoo_cs", "k_scoo_cs_finish"] if info["scoo_colsort"] else ["k_scoo"]
+    if fmt == "csr_adaptive":
+        return ["k_csr_adaptive", "k_adaptive_finish"] if info["chunks"] else ["k_csr_adaptive"]
+    return {"csr_scalar": ["k_csr_scalar"], "csr_vector": ["k_csr_vector"], "ell": ["k_ell"], "coo": ["k_coo"],
+            "hyb": ["k_ell", "k_coo"] if info["coo_elements"] else ["k_ell"],
+            "hyb_atomic": ["k_hyb_atomic"]}.get(fmt)
 
 
 def time_plan(mfp, plan, xd, yd, reps, flush, timer):
@@ -443,6 +462,24 @@ def main():
     mean_ms = statistics.fmean(ms)
     value = nbytes / (mean_ms * 1e-3) / 1e9
 
+    # roofline of the dominant kernel: the same flushed steps again, each with
+    # an event after every kernel of the SpMV (mfp_plan_kernel_times)
+    per = []
+    for _ in range(args.steps):
+        if flush:
+            mfp.flush_cache(1 << 30)
+        per.append(plan.kernel_times(xd, yd))
+    nk = min(len(k) for k in per)
+    kernel_ms = [statistics.fmean(k[i] for k in per) for i in range(nk)]
+    dom = max(range(nk), key=lambda i: kernel_ms[i])
+    # a lone kernel does the whole SpMV; where a second kernel writes y
+    # (partial slots or chunk sums added), the dominant one is credited with
+    # the compulsory reads only (matrix and x)
+    y_bytes = rows * s
+    dom_bytes = nbytes if nk == 1 else nbytes - y_bytes
+    dom_names = None if args.values else kernel_names(best, plan.info())
+    achieved = dom_bytes / (kernel_ms[dom] * 1e-3) / 1e9
+
     traffic = None
     vtag = args.values or args.dtype
     tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{vtag}_{best}.json")
@@ -486,11 +523,15 @@ def main():
         "formats": table,
         "roofline": {
             "bound": "hbm",
-            "achieved": round(value, 2),
+            "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(value / HBM_PEAK_GBS, 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "kernel": dom_names[dom] if dom_names and len(dom_names) == nk else f"kernel {dom + 1} of {nk}",
+            "ms_per_launch": round(kernel_ms[dom], 5),
+            "algorithmic_bytes_per_launch": dom_bytes,
+            "kernel_ms": [round(t, 5) for t in kernel_ms],
         },
     }
     # host-to-host rate: upload CSR and x, build the plan, one SpMV, download y

@@ -188,6 +188,17 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
                     mfp_stream s);
 /* y = A x on stream s (no host sync; graph-capturable). */
 int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream s);
+/* One mfp_plan_spmv with an event recorded before it and after each kernel
+ * it launches; waits for the last event.  ms[i] (i < min(*n_kernels,
+ * max_kernels)) is the time from the previous event to the end of kernel i,
+ * and *n_kernels is the number of kernels (at most 8; 1 with an empty
+ * interval when nothing is launched).  A plan's SpMV is one kernel for most
+ * formats, two for the column-sorted SCOO (main, finish) and for
+ * CSR-Adaptive with long-row chunks.  Splits the event-pair kernel timing of
+ * gpu_matrix_multiplier.cu:92-112 per kernel; not for graph capture, not
+ * re-entrant on one host thread. */
+int mfp_plan_kernel_times(mfp_plan p, const void *x, void *y, mfp_stream s, float *ms, int max_kernels,
+                          int *n_kernels);
 /* Y = A X for k right-hand sides (k in {1,2,4,8,16}); X is cols x k and Y
  * rows x k, row-major; X is 16-byte aligned when k values make a multiple
  * of 16 bytes.  k > 1 runs on CSR-vector plans (f32 / f64; each column is

@@ -150,6 +150,7 @@ SIGNATURES = {
     "mfp_tuning_default": (i32, [vp, i32, P(tuning)]),
     "mfp_plan_create": (i32, [P(vp), vp, i32, P(tuning), vp]),
     "mfp_plan_spmv": (i32, [vp, vp, vp, vp]),
+    "mfp_plan_kernel_times": (i32, [vp, vp, vp, vp, P(C.c_float), i32, P(i32)]),
     "mfp_plan_spmm": (i32, [vp, u32, vp, vp, vp]),
     "mfp_batch_create": (i32, [P(vp), P(vp), u32, u32, P(tuning), vp]),
     "mfp_batch_info": (i32, [vp, P(u32), P(u32), P(u64), vp, vp]),

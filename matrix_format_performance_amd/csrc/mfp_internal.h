@@ -41,6 +41,24 @@ const std::string &last_error();
     }                                                                           \
   } while (0)
 
+// Kernel marks for mfp_plan_kernel_times: while one is installed on the
+// calling thread, the check that follows every kernel launch records an
+// event on the marks' stream, so that consecutive events bracket consecutive
+// kernels of one SpMV.
+struct kernel_marks {
+  static constexpr int kMax = 8;
+  hipStream_t s = nullptr;
+  hipEvent_t ev[kMax + 1] = {};
+  int n = 0; // events recorded after ev[0]
+};
+extern thread_local kernel_marks *g_kernel_marks;
+
+inline void mark_kernel_end()
+{
+  kernel_marks *k = g_kernel_marks;
+  if (k && k->n < kernel_marks::kMax) (void)hipEventRecord(k->ev[1 + k->n++], k->s);
+}
+
 inline size_t dtype_size(int dtype)
 {
   switch (dtype) {

@@ -622,6 +622,51 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
   return MFP_ERR_ARG;
 }
 
+int mfp_plan_kernel_times(mfp_plan p, const void *x, void *y, mfp_stream sv, float *ms, int max_kernels,
+                          int *n_kernels)
+{
+  MFP_ARG(p && x && y && ms && n_kernels, "null pointer");
+  MFP_ARG(max_kernels > 0, "max_kernels must be positive");
+  MFP_ARG(!g_kernel_marks, "mfp_plan_kernel_times does not nest");
+  kernel_marks km;
+  km.s = as_stream(sv);
+  int made = 0, st = MFP_OK;
+  for (; made <= kernel_marks::kMax; made++)
+    if (hipEventCreate(&km.ev[made]) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      st = MFP_ERR_HIP;
+      break;
+    }
+  if (st == MFP_OK && hipEventRecord(km.ev[0], km.s) != hipSuccess) {
+    set_error("hipEventRecord failed");
+    st = MFP_ERR_HIP;
+  }
+  if (st == MFP_OK) {
+    g_kernel_marks = &km;
+    st = mfp_plan_spmv(p, x, y, sv);
+    g_kernel_marks = nullptr;
+  }
+  if (st == MFP_OK && km.n == 0) { // nothing launched (no rows): one empty interval
+    if (hipEventRecord(km.ev[1], km.s) != hipSuccess) st = MFP_ERR_HIP;
+    km.n = 1;
+  }
+  if (st == MFP_OK && hipEventSynchronize(km.ev[km.n]) != hipSuccess) {
+    set_error("hipEventSynchronize failed");
+    st = MFP_ERR_HIP;
+  }
+  if (st == MFP_OK) {
+    const int n = std::min(km.n, max_kernels);
+    for (int i = 0; i < n && st == MFP_OK; i++)
+      if (hipEventElapsedTime(&ms[i], km.ev[i], km.ev[i + 1]) != hipSuccess) {
+        set_error("hipEventElapsedTime failed");
+        st = MFP_ERR_HIP;
+      }
+    *n_kernels = km.n;
+  }
+  for (int i = 0; i < made; i++) (void)hipEventDestroy(km.ev[i]);
+  return st;
+}
+
 int mfp_plan_get_info(mfp_plan p, mfp_plan_info *o)
 {
   MFP_ARG(p && o, "null pointer");

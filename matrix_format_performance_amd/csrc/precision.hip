@@ -453,7 +453,11 @@ __global__ void k_ell_lp_build(uint32_t rows, uint32_t stride, const uint32_t *_
   }
 }
 
-static int launched_lp() { return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP); }
+static int launched_lp()
+{
+  mark_kernel_end();
+  return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
+}
 
 int launch_spmv_f64(uint32_t rows, const uint32_t *rp, const uint32_t *ci, const void *vals, int vdt,
                     const void *x, int xdt, double *y64, double *absdot, hipStream_t s)

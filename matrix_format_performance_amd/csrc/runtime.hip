@@ -12,6 +12,7 @@
 namespace mfp {
 
 static thread_local std::string g_last_error;
+thread_local kernel_marks *g_kernel_marks = nullptr;
 void set_error(const std::string &msg) { g_last_error = msg; }
 const std::string &last_error() { return g_last_error; }
 

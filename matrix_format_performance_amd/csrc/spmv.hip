@@ -956,7 +956,11 @@ __global__ __launch_bounds__(CS_FIN_ROWS) void k_scoo_cs_finish_k(uint32_t rows,
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static int launched() { return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP); }
+static int launched()
+{
+  mark_kernel_end();
+  return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
+}
 
 template <typename T>
 static int csr_vector_t(uint32_t lanes, uint32_t rows, const uint32_t *rp, const uint32_t *ci,

@@ -248,6 +248,14 @@ class Plan:
     def spmv(self, x: DeviceArray, y: DeviceArray, stream=None) -> None:
         L.call("mfp_plan_spmv", self.handle, x.ptr, y.ptr, stream)
 
+    def kernel_times(self, x: DeviceArray, y: DeviceArray, stream=None) -> list:
+        """One SpMV with an event after each kernel it launches; returns the
+        per-kernel times in ms (waits for the SpMV)."""
+        ms = (C.c_float * 8)()
+        n = C.c_int(0)
+        L.call("mfp_plan_kernel_times", self.handle, x.ptr, y.ptr, stream, ms, 8, C.byref(n))
+        return [float(ms[i]) for i in range(min(n.value, 8))]
+
     def spmm(self, k: int, X: DeviceArray, Y: DeviceArray, stream=None) -> None:
         """Y (rows x k) = A X (cols x k), row-major."""
         L.call("mfp_plan_spmm", self.handle, k, X.ptr, Y.ptr, stream)

@@ -312,3 +312,23 @@ def test_ell_budget_refusal(gpu):
     with pytest.raises(gpu.MfpBudgetError):
         gpu.Plan(A, "ell", budget_bytes=1 << 20)
     A.destroy()
+
+
+@pytest.mark.parametrize("fmt,tuning,kernels", [
+    ("csr_scalar", {}, 1),
+    ("scoo", {"flags": 4, "scoo_slice_size": 256, "scoo_pieces": 3}, 2),
+])
+def test_plan_kernel_times(gpu, fmt, tuning, kernels):
+    # one SpMV with an event after each kernel: one interval per kernel, and
+    # the same y as a plain SpMV
+    rp, ci, va, cols = case("powerlaw_20k", "float32")
+    announce(f"kernel times fmt={fmt} tuning={tuning}")
+    x = gpu.generate_vector(7, cols, np.float32)
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, fmt, **tuning)
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(rp.size - 1, np.float32)
+    yd.fill(np.nan)
+    ms = plan.kernel_times(xd, yd)
+    assert len(ms) == kernels and all(t > 0 for t in ms)
+    _tol_check(yd.to_host(), rp, ci, va, x)
