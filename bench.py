@@ -483,6 +483,9 @@ def main():
     traffic = None
     vtag = args.values or args.dtype
     tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{vtag}_{best}.json")
+    if not os.path.exists(tpath) and best == "hyb" and dom_names == ["k_ell"]:
+        # a hybrid whose COO part is empty runs the ELL kernel on the ELL layout
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{vtag}_ell.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
