@@ -1,0 +1,162 @@
+"""GPU parity at the full sizes of the benchmark workloads.
+
+The oracle's C restatement finishes these sizes in well under a second, so
+the deterministic kernels stay bit-exact against its order emulators here
+too.  On top of that, size-independent properties:
+
+- 2D 5-point Laplacian 256 x 256 (65,536 rows): with x = 1, y is 0 inside,
+  1 on the edges and 2 at the corners, exactly, in any summation order, so
+  every format must match it bit for bit;
+- power-law 1,048,576 rows (11.9M nnz): every feasible format within the
+  per-row tolerance of the sequential CPU answer, linearity of the
+  column-sorted SCOO, and the ELL budget refusal (width about 50,000);
+- 3D 27-point 128^3 (2,097,152 rows, 55.7M nnz): ELL and CSR-vector
+  bit-exact, the column-sorted SCOO within tolerance, and x = 1 giving the
+  row sums.
+"""
+import functools
+
+import numpy as np
+import pytest
+
+import oracle
+from gpu_cases import announce
+
+pytestmark = pytest.mark.gpu
+
+RTOL = {np.dtype(np.float32): 1e-5, np.dtype(np.float64): 1e-12}
+COLSORT = {"flags": 4, "scoo_slice_size": 16384, "scoo_pieces": 0, "scoo_threads": 1024}
+
+
+@functools.lru_cache(maxsize=None)
+def _workload(name, dtype):
+    import matrix_format_performance_amd as mfp
+    return mfp.workload(name, np.dtype(dtype))
+
+
+def _spmv(gpu, A, fmt, x, rows, **tuning):
+    plan = gpu.Plan(A, fmt, **tuning)
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(rows, x.dtype)
+    yd.fill(np.nan)
+    plan.spmv(xd, yd)
+    gpu.synchronize()
+    y = yd.to_host()
+    info = plan.info()
+    plan.destroy()
+    return y, info
+
+
+def _tol(y, rp, ci, va, x):
+    assert not np.isnan(y).any(), "unwritten rows"
+    y_ref = oracle.spmv_seq(rp, ci, va, x)
+    _, absdot = oracle.spmv_f64(rp, ci, va, x)
+    rtol = RTOL[va.dtype]
+    err = oracle.tolerance_check(y, y_ref, absdot, rtol)
+    assert err <= rtol, err
+
+
+FORMATS_C1 = [("csr_scalar", {}), ("csr_vector", {"lanes": 4}), ("csr_adaptive", {}), ("ell", {}),
+              ("coo", {}), ("scoo", {}), ("scoo", {"flags": 4, "scoo_slice_size": 4096}), ("hyb", {}),
+              ("hyb_atomic", {})]
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("fmt,tuning", FORMATS_C1)
+def test_laplace_256_ones_closed_form(gpu, fmt, tuning, dtype):
+    rp, ci, va = _workload("stencil2d_256", dtype)
+    rows = rp.size - 1
+    assert rows == 65536 and ci.size == 326656
+    announce(f"full size stencil2d_256 x=1 fmt={fmt} tuning={tuning} dtype={dtype}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, rows)
+    y, _ = _spmv(gpu, A, fmt, np.ones(rows, np.dtype(dtype)), rows, **tuning)
+    A.destroy()
+    i, j = np.divmod(np.arange(rows), 256)
+    edges = (i == 0).astype(int) + (i == 255) + (j == 0) + (j == 255)
+    assert np.array_equal(y, edges.astype(np.dtype(dtype)))
+
+
+@pytest.fixture(scope="module", params=["float32", "float64"])
+def powerlaw(request, gpu):
+    dtype = np.dtype(request.param)
+    rp, ci, va = _workload("powerlaw_1m", dtype)
+    rows = rp.size - 1
+    announce(f"full size powerlaw_1m dtype={dtype} nnz={ci.size}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, rows)
+    x = gpu.generate_vector(1, rows, dtype)
+    yield rp, ci, va, x, A
+    A.destroy()
+
+
+def test_powerlaw_1m_deterministic_kernels_bit_exact(gpu, powerlaw):
+    rp, ci, va, x, A = powerlaw
+    rows = rp.size - 1
+    y, _ = _spmv(gpu, A, "csr_scalar", x, rows)
+    assert np.array_equal(y, oracle.spmv_seq(rp, ci, va, x))
+    y, _ = _spmv(gpu, A, "csr_vector", x, rows, lanes=32)
+    assert np.array_equal(y, oracle.spmv_lanes(rp, ci, va, x, 32))
+    y, info = _spmv(gpu, A, "csr_adaptive", x, rows)
+    E = gpu.engine
+    ref = oracle.spmv_adaptive(rp, ci, va, x, info["nnz_per_wg"], 256, E.DEFAULT_LONG_ROW, E.DEFAULT_CHUNK)
+    assert np.array_equal(y, ref)
+
+
+@pytest.mark.parametrize("fmt,tuning", [("scoo", COLSORT), ("coo", {}), ("scoo", {}), ("hyb", {"hyb_width": 16}),
+                                        ("hyb_atomic", {"hyb_width": 16})])
+def test_powerlaw_1m_within_tolerance(gpu, powerlaw, fmt, tuning):
+    rp, ci, va, x, A = powerlaw
+    announce(f"full size powerlaw_1m fmt={fmt} tuning={tuning} dtype={va.dtype}")
+    y, _ = _spmv(gpu, A, fmt, x, rp.size - 1, **tuning)
+    _tol(y, rp, ci, va, x)
+
+
+def test_powerlaw_1m_column_sorted_is_linear(gpu, powerlaw):
+    # A (x1 + x2) = A x1 + A x2 up to rounding: the bound is three times the
+    # per-row tolerance of each product, relative to sum_j |a_ij| (|x1| + |x2|)
+    rp, ci, va, x1, A = powerlaw
+    rows = rp.size - 1
+    x2 = gpu.generate_vector(2, rows, va.dtype)
+    y1, _ = _spmv(gpu, A, "scoo", x1, rows, **COLSORT)
+    y2, _ = _spmv(gpu, A, "scoo", x2, rows, **COLSORT)
+    y12, _ = _spmv(gpu, A, "scoo", (x1 + x2).astype(va.dtype), rows, **COLSORT)
+    _, ad = oracle.spmv_f64(rp, ci, va, (np.abs(x1.astype(np.float64)) + np.abs(x2)).astype(va.dtype))
+    diff = np.abs(y12.astype(np.float64) - (y1.astype(np.float64) + y2))
+    scale = np.maximum(ad, np.finfo(np.float64).tiny)
+    assert np.max(diff / scale) <= 3 * RTOL[va.dtype]
+
+
+def test_powerlaw_1m_ell_refused_by_budget(gpu, powerlaw):
+    # the longest row is about 50,000 entries: an ELL of 1M rows x that width
+    # is refused before anything is allocated
+    rp, ci, va, x, A = powerlaw
+    with pytest.raises(gpu.MfpBudgetError):
+        gpu.Plan(A, "ell")
+
+
+@pytest.fixture(scope="module")
+def stencil3d(gpu):
+    rp, ci, va = _workload("stencil3d_128", "float32")
+    rows = rp.size - 1
+    assert rows == 2097152 and ci.size == 55742968
+    announce(f"full size stencil3d_128 nnz={ci.size}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, rows)
+    yield rp, ci, va, A
+    A.destroy()
+
+
+def test_stencil3d_128_bit_exact_and_row_sums(gpu, stencil3d):
+    rp, ci, va, A = stencil3d
+    rows = rp.size - 1
+    x = gpu.generate_vector(3, rows, np.float32)
+    y, info = _spmv(gpu, A, "ell", x, rows)
+    w, ed, ec = oracle.ell_from_csr(rp, ci, va)
+    assert info["ell_width"] == w == 27
+    assert np.array_equal(y, oracle.spmv_ell(rows, w, ec, ed, x))
+    y, _ = _spmv(gpu, A, "csr_vector", x, rows, lanes=32)
+    assert np.array_equal(y, oracle.spmv_lanes(rp, ci, va, x, 32))
+    y, _ = _spmv(gpu, A, "scoo", x, rows, **COLSORT)
+    _tol(y, rp, ci, va, x)
+    # x = 1: the row sums of the values
+    ones = np.ones(rows, np.float32)
+    y, _ = _spmv(gpu, A, "hyb", ones, rows, hyb_width=27)
+    _tol(y, rp, ci, va, ones)
