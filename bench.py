@@ -55,31 +55,11 @@ WORKLOADS = {
 }
 
 
-def batch_members(mfp, count=512, seed=0):
-    """The batch workload's matrices (host CSR, FP32)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        rows = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
-        fam = ("stencil2d", "stencil3d", "uniform", "powerlaw")[i % 4]
-        if fam == "stencil2d":
-            spec = mfp.GenSpec("stencil2d", values="uniform", n=max(2, int(np.sqrt(rows))), seed=i)
-        elif fam == "stencil3d":
-            spec = mfp.GenSpec("stencil3d", values="normal", n=max(2, int(round(rows ** (1 / 3)))), seed=i)
-        elif fam == "uniform":
-            spec = mfp.GenSpec("uniform", values="uniform", rows=rows, nnz_lo=4, nnz_hi=64, seed=i)
-        else:
-            spec = mfp.GenSpec("powerlaw", values="uniform", rows=rows, nnz_lo=1, nnz_hi=min(rows // 2, 10000),
-                               alpha=float(rng.uniform(1.5, 3.0)), seed=i)
-        out.append(mfp.generate(spec, np.float32))
-    return out
-
-
 def run_batch(mfp, args, timer):
     """One step = one replay of the captured persistent launch over all 512
     matrices (k right-hand sides); value = algorithmic bytes / mean step."""
     t0 = time.time()
-    members = batch_members(mfp)
+    members = mfp.engine.batch_workload()
     dms = [mfp.DeviceMatrix.from_csr(rp, ci, va) for rp, ci, va in members]
     # lanes per row for the k > 1 launch: the mean row length rounded up to a
     # power of two for every row, or each row's own length (binned, lanes 0).

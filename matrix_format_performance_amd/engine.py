@@ -465,6 +465,28 @@ def generate_vector(seed: int, n: int, dtype=np.float32) -> np.ndarray:
 
 
 # Benchmark workloads: 2D 5-point 256x256, power-law 1M rows, 3D 27-point 128^3
+def batch_workload(count: int = 512, seed: int = 0) -> list:
+    """The 512-matrix batch workload (host CSR triples, FP32): rows log-uniform
+    in [1k, 200k], families in turn 2D 5-point and 3D 27-point stencils,
+    uniform 4-64 nnz per row, and power-law rows (alpha U[1.5, 3])."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        rows = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+        fam = ("stencil2d", "stencil3d", "uniform", "powerlaw")[i % 4]
+        if fam == "stencil2d":
+            spec = GenSpec("stencil2d", values="uniform", n=max(2, int(np.sqrt(rows))), seed=i)
+        elif fam == "stencil3d":
+            spec = GenSpec("stencil3d", values="normal", n=max(2, int(round(rows ** (1 / 3)))), seed=i)
+        elif fam == "uniform":
+            spec = GenSpec("uniform", values="uniform", rows=rows, nnz_lo=4, nnz_hi=64, seed=i)
+        else:
+            spec = GenSpec("powerlaw", values="uniform", rows=rows, nnz_lo=1, nnz_hi=min(rows // 2, 10000),
+                           alpha=float(rng.uniform(1.5, 3.0)), seed=i)
+        out.append(generate(spec, np.float32))
+    return out
+
+
 def workload(name: str, dtype=np.float32, seed: int = 0):
     if name == "stencil2d_256":
         return generate(GenSpec("stencil2d", values="laplace", n=256, seed=seed), dtype)

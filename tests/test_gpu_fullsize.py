@@ -12,7 +12,10 @@ too.  On top of that, size-independent properties:
   column-sorted SCOO, and the ELL budget refusal (width about 50,000);
 - 3D 27-point 128^3 (2,097,152 rows, 55.7M nnz): ELL and CSR-vector
   bit-exact, the column-sorted SCOO within tolerance, and x = 1 giving the
-  row sums.
+  row sums;
+- the 512-matrix batch (about 20M rows, 430M nnz) for k = 1, 4 and 16:
+  sampled members' blocks against the oracle on the member alone, and a
+  checksum of checksums over all rows.
 """
 import functools
 
@@ -160,3 +163,53 @@ def test_stencil3d_128_bit_exact_and_row_sums(gpu, stencil3d):
     ones = np.ones(rows, np.float32)
     y, _ = _spmv(gpu, A, "hyb", ones, rows, hyb_width=27)
     _tol(y, rp, ci, va, ones)
+
+
+def test_batch512_member_blocks_and_checksum(gpu):
+    # the 512-matrix batch (about 20M rows, 430M nnz) as the benchmark runs
+    # it: k = 1 through the column-sorted SCOO, k = 4 and 16 through the
+    # 32-lane CSR-vector SpMM.  Sampled members' blocks of Y against the
+    # oracle on the member alone (bit-exact for the CSR-vector order), and
+    # for every k a checksum of checksums: sum_i Y[i, c] = sum_j colsum_j X[j, c]
+    members = gpu.engine.batch_workload()
+    assert len(members) == 512
+    announce(f"full size batch512 rows={sum(m[0].size - 1 for m in members)} "
+             f"nnz={sum(m[1].size for m in members)}")
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    batch = gpu.Batch(dms, lanes=32, flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=16384,
+                      scoo_pieces=2, scoo_threads=1024)
+    for d in dms:
+        d.destroy()
+    r_off, c_off = batch.row_offsets, batch.col_offsets
+    colsum = [np.bincount(ci, weights=va.astype(np.float64), minlength=rp.size - 1) for rp, ci, va in members]
+    abscol = [np.bincount(ci, weights=np.abs(va.astype(np.float64)), minlength=rp.size - 1)
+              for rp, ci, va in members]
+    sample = [0, 1, 2, 3, 4, 5, 6, 7, int(np.argmax([m[1].size for m in members]))]
+    for k in (1, 4, 16):
+        announce(f"full size batch512 k={k}")
+        X = np.random.default_rng(k).uniform(-1, 1, (batch.cols, k)).astype(np.float32)
+        Xd = gpu.DeviceArray.from_host(X.ravel())
+        Yd = gpu.DeviceArray(batch.rows * k, np.float32)
+        Yd.fill(np.nan)
+        batch.spmm(k, Xd, Yd)
+        gpu.synchronize()
+        Y = Yd.to_host().reshape(-1, k)
+        Xd.free()
+        Yd.free()
+        assert not np.isnan(Y).any()
+        for j in sample:
+            rp, ci, va = members[j]
+            Yj = Y[r_off[j]:r_off[j + 1]]
+            Xj = np.ascontiguousarray(X[c_off[j]:c_off[j + 1]])
+            for c in range(k):
+                xc = np.ascontiguousarray(Xj[:, c])
+                if k == 1:
+                    _tol(np.ascontiguousarray(Yj[:, c]), rp, ci, va, xc)
+                else:
+                    assert np.array_equal(Yj[:, c], oracle.spmv_lanes(rp, ci, va, xc, 32)), (j, c)
+        for c in range(k):
+            want = sum(float(colsum[j] @ X[c_off[j]:c_off[j + 1], c].astype(np.float64)) for j in range(512))
+            bound = sum(float(abscol[j] @ np.abs(X[c_off[j]:c_off[j + 1], c].astype(np.float64)))
+                        for j in range(512))
+            assert abs(float(Y[:, c].astype(np.float64).sum()) - want) <= 1e-5 * bound, (k, c)
+    batch.destroy()
