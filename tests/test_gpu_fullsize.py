@@ -11,8 +11,8 @@ too.  On top of that, size-independent properties:
   per-row tolerance of the sequential CPU answer, linearity of the
   column-sorted SCOO, and the ELL budget refusal (width about 50,000);
 - 3D 27-point 128^3 (2,097,152 rows, 55.7M nnz): ELL and CSR-vector
-  bit-exact, the column-sorted SCOO within tolerance, and x = 1 giving the
-  row sums;
+  bit-exact, the column-sorted SCOO within tolerance, x = 1 giving the row
+  sums, and the 16-bit value forms against FP64 of the stored values;
 - the 512-matrix batch (about 20M rows, 430M nnz) for k = 1, 4 and 16:
   sampled members' blocks against the oracle on the member alone, and a
   checksum of checksums over all rows.
@@ -213,3 +213,30 @@ def test_batch512_member_blocks_and_checksum(gpu):
                         for j in range(512))
             assert abs(float(Y[:, c].astype(np.float64).sum()) - want) <= 1e-5 * bound, (k, c)
     batch.destroy()
+
+
+@pytest.mark.parametrize("vt", ["bf16", "f16"])
+def test_stencil3d_128_16bit_values(gpu, stencil3d, vt):
+    # the 16-bit sliced ELL (Dot2-compensated FP32 sums) and the column-sorted
+    # SCOO (exact products added in double) at full size, against FP64 of the
+    # stored values; the conversion itself is round-to-nearest-even
+    rp, ci, va, A = stencil3d
+    rows = rp.size - 1
+    announce(f"full size stencil3d_128 {vt}")
+    B = A.convert(vt)
+    _, _, bits = B.download()
+    if vt == "bf16":
+        from oracle.selector import bf16_round  # rounded to BF16, kept as float32
+        stored = bf16_round(va)
+        assert np.array_equal(bits, (stored.view(np.uint32) >> 16).astype(np.uint16))
+    else:
+        assert np.array_equal(bits, va.astype(np.float16).view(np.uint16))
+        stored = va.astype(np.float16).astype(np.float32)
+    x = gpu.generate_vector(4, rows, np.float32)
+    y64, ad = oracle.spmv_f64(rp, ci, stored, x)
+    for fmt, kw, bound in (("ell", {}, 2.5e-7), ("scoo", COLSORT, 1e-6)):
+        y, _ = _spmv(gpu, B, fmt, x, rows, **kw)
+        assert not np.isnan(y).any()
+        err = oracle.tolerance_check(y, y64, ad, 1.0)
+        assert err <= bound, (fmt, err)
+    B.destroy()
