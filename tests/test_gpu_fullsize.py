@@ -9,7 +9,8 @@ too.  On top of that, size-independent properties:
   every format must match it bit for bit;
 - power-law 1,048,576 rows (11.9M nnz): every feasible format within the
   per-row tolerance of the sequential CPU answer, linearity of the
-  column-sorted SCOO, and the ELL budget refusal (width about 50,000);
+  column-sorted SCOO, the FP64 GPU oracle and residual, SpMM with 4 and 16
+  right-hand sides, and the ELL budget refusal (width about 50,000);
 - 3D 27-point 128^3 (2,097,152 rows, 55.7M nnz): ELL and CSR-vector
   bit-exact, the column-sorted SCOO within tolerance, x = 1 giving the row
   sums, and the 16-bit value forms against FP64 of the stored values;
@@ -126,6 +127,54 @@ def test_powerlaw_1m_column_sorted_is_linear(gpu, powerlaw):
     diff = np.abs(y12.astype(np.float64) - (y1.astype(np.float64) + y2))
     scale = np.maximum(ad, np.finfo(np.float64).tiny)
     assert np.max(diff / scale) <= 3 * RTOL[va.dtype]
+
+
+def test_powerlaw_1m_fp64_oracle_and_residual(gpu, powerlaw):
+    # the GPU FP64 SpMV against the CPU one, and the residual of the
+    # column-sorted SCOO measured against it
+    rp, ci, va, x, A = powerlaw
+    xd = gpu.DeviceArray.from_host(x)
+    y64, ad = A.oracle_f64(xd)
+    gpu.synchronize()
+    ry, rad = oracle.spmv_f64(rp, ci, va, x)
+    assert np.max(np.abs(y64.to_host() - ry) / np.maximum(rad, 1e-300)) < 1e-14
+    # sum_j |a_ij x_j| of positive terms in two orders: at most (n - 1) 2^-53
+    # apart, relative, for a row of n entries (up to about 50,000 here)
+    n_max = int(np.max(np.diff(rp.astype(np.int64))))
+    assert np.allclose(ad.to_host(), rad, rtol=n_max * 2.0 ** -53, atol=0)
+    plan = gpu.Plan(A, "scoo", **COLSORT)
+    yd = gpu.DeviceArray(rp.size - 1, va.dtype)
+    plan.spmv(xd, yd)
+    rep = gpu.engine.residual(yd, y64, ad)
+    assert rep["max_row_err"] <= RTOL[va.dtype]
+    plan.destroy()
+
+
+@pytest.mark.parametrize("k", [4, 16])
+def test_powerlaw_1m_spmm(gpu, powerlaw, k):
+    # k right-hand sides: the 32-lane CSR-vector SpMM is bit-exact per column
+    # against the CSR-vector order; the column-sorted SpMM (4,096-row slices
+    # hold 4 columns of doubles, 1,024-row slices 16) is within tolerance
+    rp, ci, va, x, A = powerlaw
+    rows = rp.size - 1
+    announce(f"full size powerlaw_1m spmm k={k} dtype={va.dtype}")
+    X = np.random.default_rng(k).uniform(-1, 1, (rows, k)).astype(va.dtype)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray(rows * k, va.dtype)
+    for fmt, kw in (("csr_vector", {"lanes": 32}),
+                    ("scoo", {"flags": 4, "scoo_slice_size": 4096 if k == 4 else 1024, "scoo_pieces": 0})):
+        plan = gpu.Plan(A, fmt, **kw)
+        Yd.fill(np.nan)
+        plan.spmm(k, Xd, Yd)
+        gpu.synchronize()
+        Y = Yd.to_host().reshape(-1, k)
+        plan.destroy()
+        for j in range(k):
+            xj = np.ascontiguousarray(X[:, j])
+            if fmt == "csr_vector":
+                assert np.array_equal(Y[:, j], oracle.spmv_lanes(rp, ci, va, xj, 32)), j
+            else:
+                _tol(np.ascontiguousarray(Y[:, j]), rp, ci, va, xj)
 
 
 def test_powerlaw_1m_ell_refused_by_budget(gpu, powerlaw):
