@@ -15,8 +15,9 @@ too.  On top of that, size-independent properties:
   bit-exact, the column-sorted SCOO within tolerance, x = 1 giving the row
   sums, and the 16-bit value forms against FP64 of the stored values;
 - the 512-matrix batch (about 20M rows, 430M nnz) for k = 1, 4 and 16:
-  sampled members' blocks against the oracle on the member alone, and a
-  checksum of checksums over all rows.
+  sampled members' blocks against the oracle on the member alone, a
+  checksum of checksums over all rows, and HIP-graph replays equal to the
+  eager launch.
 """
 import functools
 
@@ -243,6 +244,15 @@ def test_batch512_member_blocks_and_checksum(gpu):
         batch.spmm(k, Xd, Yd)
         gpu.synchronize()
         Y = Yd.to_host().reshape(-1, k)
+        if k > 1:
+            # the captured launch, replayed twice, gives the eager result bit for bit
+            g = batch.graph(k, Xd, Yd)
+            for _ in range(2):
+                Yd.fill(np.nan)
+                g.launch()
+                gpu.synchronize()
+                assert np.array_equal(Yd.to_host().reshape(-1, k), Y)
+            g.destroy()
         Xd.free()
         Yd.free()
         assert not np.isnan(Y).any()
