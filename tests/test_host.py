@@ -48,6 +48,17 @@ def test_no_gpu_calls_fail_cleanly():
         lib.mfp_free(p)
 
 
+def test_kernel_times_rejects_null_arguments():
+    # argument checks come before any device work
+    import ctypes
+    lib = L.load()
+    ms = (ctypes.c_float * 8)()
+    n = ctypes.c_int(-1)
+    assert lib.mfp_plan_kernel_times(None, None, None, None, ms, 8, ctypes.byref(n)) == L.MFP_ERR_ARG
+    assert n.value == -1
+    assert b"null" in lib.mfp_last_error()
+
+
 def test_package_fails_loudly_without_library(tmp_path):
     # a copy of the package without libmfp.so must raise on first use
     import shutil
