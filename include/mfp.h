@@ -220,7 +220,9 @@ typedef struct mfp_graph_s *mfp_graph;
  * next power of two of its length, from 2 to 64 lanes (each column is then
  * summed in the CSR-vector order with its row's lane count); 1 for the
  * column-sorted SCOO SpMM over the assembled matrix (one plan per k, built by
- * the first mfp_batch_spmm or mfp_batch_graph_create with that k). */
+ * the first mfp_batch_spmm or mfp_batch_graph_create with that k: the
+ * largest power-of-two slice whose k doubles per row fit in 160 KB of LDS,
+ * one work item per slice and column band, 1,024 threads). */
 /* k1_tuning: the plan for k = 1 over the assembled matrix: CSR-Adaptive
  * (NULL = its defaults), or the column-sorted SCOO when its flags hold
  * MFP_TUNE_SCOO_COLSORT (two kernels, not bit-exact to a row order). */
