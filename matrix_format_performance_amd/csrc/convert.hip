@@ -899,6 +899,35 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
       const uint64_t ka = a - first[a], kb = b - first[b];
       return (2 * ka + 1) * pieces[b] < (2 * kb + 1) * pieces[a];
     });
+    // The kernel maps workgroups to items with xcd_block: XCD x runs one
+    // contiguous range of this order, first those of its items that fill its
+    // CUs, then the rest as CUs free up.  Items of ordinary slices take about
+    // twice as long as long-row items, so each range gets an even share of
+    // them, placed first, then long-row items (both in band-centre order, so
+    // each XCD still works on one band of x).  Left to the band-centre order
+    // alone, one XCD can receive more ordinary items than it has CUs and run
+    // two of them back to back while others idle.
+    if (out->NI >= 32) {
+      std::vector<uint32_t> ro, lo;
+      for (uint32_t i : order) (items[i].x < S_reg ? ro : lo).push_back(i);
+      const uint32_t nr = (uint32_t)ro.size(), nl = (uint32_t)lo.size();
+      const uint32_t q = out->NI / 8, rem = out->NI % 8;
+      uint32_t ri = 0, li = 0, p0 = 0;
+      for (uint32_t x = 0; x < 8; x++) {
+        const uint32_t len = q + (x < rem ? 1u : 0u);
+        const uint32_t share = (uint32_t)((uint64_t)nr * (x + 1) / 8 - (uint64_t)nr * x / 8);
+        uint32_t rx = std::min({share, len, nr - ri});
+        const uint32_t lx = std::min(len - rx, nl - li);
+        rx += std::min(len - rx - lx, nr - ri - rx); // no long-row items left: more ordinary ones
+        for (uint32_t k = 0; k < rx; k++) order[p0++] = ro[ri++];
+        for (uint32_t k = 0; k < lx; k++) order[p0++] = lo[li++];
+      }
+      // every range is filled exactly: the ranges' lengths add up to NI
+      if (p0 != out->NI || ri != nr || li != nl) {
+        set_error("column-sorted SCOO: work item ordering failed");
+        return done(MFP_ERR_STATE);
+      }
+    }
     std::vector<uint4> it2(out->NI);
     std::vector<uint32_t> b2(out->NI);
     for (uint32_t i = 0; i < out->NI; i++) {
