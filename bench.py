@@ -15,9 +15,10 @@ candidate (every format and its swept tunings, each timed the same way as
 the steps), reported in config.format / config.tuning, with the best
 candidate of every format in the "formats" table.
 
-roofline: the same flushed steps again, with an event after each kernel of
-the SpMV (mfp_plan_kernel_times); the dominant kernel is the one with the
-longest mean time.  achieved = its algorithmic bytes / that mean, where a
+roofline: the same flushed steps again, timed per kernel of the SpMV
+(mfp_plan_kernel_times: the column-sorted SCOO's kernels record their own
+start and end, others an event after each kernel); the dominant kernel is
+the one with the longest mean time.  achieved = its algorithmic bytes / that mean, where a
 lone kernel is credited with B_alg and, in a two-kernel SpMV (a main kernel
 then one that adds partial slots or chunk sums into y), the dominant kernel
 with B_alg - rows*s_y (the matrix and x reads).  traffic = its HBM bytes

@@ -49,6 +49,8 @@ struct kernel_marks {
   static constexpr int kMax = 8;
   hipStream_t s = nullptr;
   hipEvent_t ev[kMax + 1] = {};
+  hipEvent_t start[kMax] = {}; // kernel i's own start, when exact[i]
+  bool exact[kMax] = {};
   int n = 0; // events recorded after ev[0]
 };
 extern thread_local kernel_marks *g_kernel_marks;
@@ -57,6 +59,19 @@ inline void mark_kernel_end()
 {
   kernel_marks *k = g_kernel_marks;
   if (k && k->n < kernel_marks::kMax) (void)hipEventRecord(k->ev[1 + k->n++], k->s);
+}
+
+// A launcher that launches through hipExtLaunchKernelGGL asks here for the
+// start / stop events of its kernel: true, with both, when marks are
+// installed (the kernel's own start and end times are then recorded).
+inline bool mark_kernel_events(hipEvent_t *start, hipEvent_t *stop)
+{
+  kernel_marks *k = g_kernel_marks;
+  if (!k || k->n >= kernel_marks::kMax) return false;
+  *start = k->start[k->n];
+  *stop = k->ev[1 + k->n];
+  k->exact[k->n++] = true;
+  return true;
 }
 
 inline size_t dtype_size(int dtype)

@@ -632,7 +632,8 @@ int mfp_plan_kernel_times(mfp_plan p, const void *x, void *y, mfp_stream sv, flo
   km.s = as_stream(sv);
   int made = 0, st = MFP_OK;
   for (; made <= kernel_marks::kMax; made++)
-    if (hipEventCreate(&km.ev[made]) != hipSuccess) {
+    if (hipEventCreate(&km.ev[made]) != hipSuccess ||
+        (made < kernel_marks::kMax && hipEventCreate(&km.start[made]) != hipSuccess)) {
       set_error("hipEventCreate failed");
       st = MFP_ERR_HIP;
       break;
@@ -657,13 +658,16 @@ int mfp_plan_kernel_times(mfp_plan p, const void *x, void *y, mfp_stream sv, flo
   if (st == MFP_OK) {
     const int n = std::min(km.n, max_kernels);
     for (int i = 0; i < n && st == MFP_OK; i++)
-      if (hipEventElapsedTime(&ms[i], km.ev[i], km.ev[i + 1]) != hipSuccess) {
+      if (hipEventElapsedTime(&ms[i], km.exact[i] ? km.start[i] : km.ev[i], km.ev[i + 1]) != hipSuccess) {
         set_error("hipEventElapsedTime failed");
         st = MFP_ERR_HIP;
       }
     *n_kernels = km.n;
   }
-  for (int i = 0; i < made; i++) (void)hipEventDestroy(km.ev[i]);
+  for (int i = 0; i <= kernel_marks::kMax; i++) {
+    if (km.ev[i]) (void)hipEventDestroy(km.ev[i]);
+    if (i < kernel_marks::kMax && km.start[i]) (void)hipEventDestroy(km.start[i]);
+  }
   return st;
 }
 

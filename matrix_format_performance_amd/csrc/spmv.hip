@@ -18,6 +18,8 @@
 #include "values16.h"
 #include "wave64.h"
 
+#include <hip/hip_ext.h>
+
 #include <type_traits>
 
 namespace mfp {
@@ -962,6 +964,12 @@ static int launched()
   return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
 }
 
+// launch status of a kernel launched with its own start / stop events
+static int launched_timed()
+{
+  return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
+}
+
 template <typename T>
 static int csr_vector_t(uint32_t lanes, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
                         const T *a, const T *x, T *y, hipStream_t s)
@@ -1170,14 +1178,31 @@ static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, c
   if (lds > 65536)
     MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T, DT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-  k_scoo_cs<T, DT><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
-                                              (const V *)l.ev, l.ec, (const T *)x, (T *)y, (T *)l.partial,
-                                              l.long_rows, l.long_mask);
-  MFP_CHECK(launched());
-  if (l.NF) {
-    k_scoo_cs_finish<T><<<l.NF, CS_FIN_ROWS, 0, s>>>(rows, l.R, l.S_reg, l.NL, l.NFR, l.fin, l.long_rows,
-                                                      l.long_mask, (const T *)l.partial, (T *)y);
+  // under mfp_plan_kernel_times both kernels record their own start and end
+  hipEvent_t e0, e1;
+  if (mark_kernel_events(&e0, &e1)) {
+    hipExtLaunchKernelGGL(k_scoo_cs<T, DT>, dim3(l.NI), dim3(threads), (uint32_t)lds, s, e0, e1, 0u, rows, l.R,
+                          l.rbits, l.S_reg, l.NL, (const uint4 *)l.items, (const uint32_t *)l.base,
+                          (const V *)l.ev, (const uint32_t *)l.ec, (const T *)x, (T *)y, (T *)l.partial,
+                          (const uint32_t *)l.long_rows, (const uint32_t *)l.long_mask);
+    MFP_CHECK(launched_timed());
+  } else {
+    k_scoo_cs<T, DT><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
+                                                (const V *)l.ev, l.ec, (const T *)x, (T *)y, (T *)l.partial,
+                                                l.long_rows, l.long_mask);
     MFP_CHECK(launched());
+  }
+  if (l.NF) {
+    if (mark_kernel_events(&e0, &e1)) {
+      hipExtLaunchKernelGGL(k_scoo_cs_finish<T>, dim3(l.NF), dim3(CS_FIN_ROWS), 0u, s, e0, e1, 0u, rows, l.R,
+                            l.S_reg, l.NL, l.NFR, (const uint4 *)l.fin, (const uint32_t *)l.long_rows,
+                            (const uint32_t *)l.long_mask, (const T *)l.partial, (T *)y);
+      MFP_CHECK(launched_timed());
+    } else {
+      k_scoo_cs_finish<T><<<l.NF, CS_FIN_ROWS, 0, s>>>(rows, l.R, l.S_reg, l.NL, l.NFR, l.fin, l.long_rows,
+                                                        l.long_mask, (const T *)l.partial, (T *)y);
+      MFP_CHECK(launched());
+    }
   }
   return MFP_OK;
 }
