@@ -970,6 +970,21 @@ static int launched_timed()
   return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
 }
 
+// Launch a kernel and check the launch.  Under mfp_plan_kernel_times the
+// kernel records its own start and end (hipExtLaunchKernelGGL events);
+// otherwise it is a plain launch followed by the usual mark.
+template <typename... P, typename... A>
+static int launch_k(void (*kernel)(P...), dim3 grid, dim3 block, uint32_t lds, hipStream_t s, A... args)
+{
+  hipEvent_t e0, e1;
+  if (mark_kernel_events(&e0, &e1)) {
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, s, e0, e1, 0u, static_cast<P>(args)...);
+    return launched_timed();
+  }
+  hipLaunchKernelGGL(kernel, grid, block, lds, s, static_cast<P>(args)...);
+  return launched();
+}
+
 template <typename T>
 static int csr_vector_t(uint32_t lanes, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
                         const T *a, const T *x, T *y, hipStream_t s)
@@ -1027,14 +1042,11 @@ static int adaptive_launch_k(const adaptive_args &g, const T *a, const T *x, T *
                                    : (const void *)k_csr_adaptive<T, K>;
   if (lds > 65536) MFP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (persistent_grid)
-    k_csr_adaptive_persistent<T, K><<<persistent_grid, AD_T, lds, s>>>(
-        items, g.rp, g.ci, a, x, y, g.blocks, g.nnz_per_wg, g.long_row, g.chunks, g.nchunks,
-        nchunks_pad, g.chunk, (T *)g.partials);
-  else
-    k_csr_adaptive<T, K><<<items, AD_T, lds, s>>>(g.rp, g.ci, a, x, y, g.blocks, g.nnz_per_wg,
-                                                  g.long_row, g.chunks, g.nchunks, nchunks_pad,
-                                                  g.chunk, (T *)g.partials);
-  return MFP_OK;
+    return launch_k(k_csr_adaptive_persistent<T, K>, dim3(persistent_grid), dim3(AD_T), (uint32_t)lds, s, items,
+                    g.rp, g.ci, a, x, y, g.blocks, g.nnz_per_wg, g.long_row, g.chunks, g.nchunks, nchunks_pad,
+                    g.chunk, (T *)g.partials);
+  return launch_k(k_csr_adaptive<T, K>, dim3(items), dim3(AD_T), (uint32_t)lds, s, g.rp, g.ci, a, x, y, g.blocks,
+                  g.nnz_per_wg, g.long_row, g.chunks, g.nchunks, nchunks_pad, g.chunk, (T *)g.partials);
 }
 
 // Dynamic LDS of k_csr_adaptive: the staged products, then the row starts.
@@ -1069,13 +1081,10 @@ static int adaptive_t(const adaptive_args &g, const T *a, const T *x, T *y, hipS
     default: set_error("nnz_per_wg too large"); return MFP_ERR_ARG;
     }
     MFP_CHECK(st);
-    MFP_CHECK(launched());
   }
-  if (g.nchunks) {
-    k_adaptive_finish<T><<<ceil_div_u32(g.nchunks, 256), 256, 0, s>>>(g.chunks, g.nchunks,
-                                                                       (const T *)g.partials, y);
-    MFP_CHECK(launched());
-  }
+  if (g.nchunks)
+    MFP_CHECK(launch_k(k_adaptive_finish<T>, dim3(ceil_div_u32(g.nchunks, 256)), dim3(256), 0, s, g.chunks,
+                       g.nchunks, (const T *)g.partials, y));
   return MFP_OK;
 }
 
@@ -1092,10 +1101,10 @@ int launch_ell(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const v
   if (rows == 0) return MFP_OK;
   const uint32_t grid = ceil_div_u32(rows, 256);
   if (dtype == MFP_F32)
-    k_ell<float><<<grid, 256, 0, s>>>(rows, w, ec, (const float *)ev, (const float *)x, (float *)y);
-  else
-    k_ell<double><<<grid, 256, 0, s>>>(rows, w, ec, (const double *)ev, (const double *)x, (double *)y);
-  return launched();
+    return launch_k(k_ell<float>, dim3(grid), dim3(256), 0, s, rows, w, ec, (const float *)ev, (const float *)x,
+                    (float *)y);
+  return launch_k(k_ell<double>, dim3(grid), dim3(256), 0, s, rows, w, ec, (const double *)ev, (const double *)x,
+                  (double *)y);
 }
 
 static uint32_t coo_grid(uint64_t n)
@@ -1178,32 +1187,12 @@ static int scoo_cs_t(uint32_t rows, const scoo_cs_layout &l, uint32_t threads, c
   if (lds > 65536)
     MFP_HIP(hipFuncSetAttribute((const void *)k_scoo_cs<T, DT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-  // under mfp_plan_kernel_times both kernels record their own start and end
-  hipEvent_t e0, e1;
-  if (mark_kernel_events(&e0, &e1)) {
-    hipExtLaunchKernelGGL(k_scoo_cs<T, DT>, dim3(l.NI), dim3(threads), (uint32_t)lds, s, e0, e1, 0u, rows, l.R,
-                          l.rbits, l.S_reg, l.NL, (const uint4 *)l.items, (const uint32_t *)l.base,
-                          (const V *)l.ev, (const uint32_t *)l.ec, (const T *)x, (T *)y, (T *)l.partial,
-                          (const uint32_t *)l.long_rows, (const uint32_t *)l.long_mask);
-    MFP_CHECK(launched_timed());
-  } else {
-    k_scoo_cs<T, DT><<<l.NI, threads, lds, s>>>(rows, l.R, l.rbits, l.S_reg, l.NL, l.items, l.base,
-                                                (const V *)l.ev, l.ec, (const T *)x, (T *)y, (T *)l.partial,
-                                                l.long_rows, l.long_mask);
-    MFP_CHECK(launched());
-  }
-  if (l.NF) {
-    if (mark_kernel_events(&e0, &e1)) {
-      hipExtLaunchKernelGGL(k_scoo_cs_finish<T>, dim3(l.NF), dim3(CS_FIN_ROWS), 0u, s, e0, e1, 0u, rows, l.R,
-                            l.S_reg, l.NL, l.NFR, (const uint4 *)l.fin, (const uint32_t *)l.long_rows,
-                            (const uint32_t *)l.long_mask, (const T *)l.partial, (T *)y);
-      MFP_CHECK(launched_timed());
-    } else {
-      k_scoo_cs_finish<T><<<l.NF, CS_FIN_ROWS, 0, s>>>(rows, l.R, l.S_reg, l.NL, l.NFR, l.fin, l.long_rows,
-                                                        l.long_mask, (const T *)l.partial, (T *)y);
-      MFP_CHECK(launched());
-    }
-  }
+  MFP_CHECK(launch_k(k_scoo_cs<T, DT>, dim3(l.NI), dim3(threads), (uint32_t)lds, s, rows, l.R, l.rbits, l.S_reg,
+                     l.NL, l.items, l.base, (const V *)l.ev, l.ec, (const T *)x, (T *)y, (T *)l.partial,
+                     l.long_rows, l.long_mask));
+  if (l.NF)
+    MFP_CHECK(launch_k(k_scoo_cs_finish<T>, dim3(l.NF), dim3(CS_FIN_ROWS), 0, s, rows, l.R, l.S_reg, l.NL, l.NFR,
+                       l.fin, l.long_rows, l.long_mask, (const T *)l.partial, (T *)y));
   return MFP_OK;
 }
 
