@@ -191,9 +191,9 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream s);
 /* One mfp_plan_spmv with an event recorded before it and after each kernel
  * it launches; waits for the last event.  ms[i] (i < min(*n_kernels,
  * max_kernels)) is kernel i's own duration where its launcher records the
- * kernel's start and end (hipExtLaunchKernelGGL events: ELL, CSR-Adaptive
- * and the column-sorted SCOO), else the time from the previous event to the
- * end of kernel i; *n_kernels is the number of kernels (at most 8; 1
+ * kernel's start and end (hipExtLaunchKernelGGL events: every FP32 / FP64
+ * format), else the time from the previous event to the end of kernel i
+ * (16-bit value plans, and the zero fill before COO and the atomic hybrid); *n_kernels is the number of kernels (at most 8; 1
  * with an empty interval when nothing is launched).  A plan's SpMV is one kernel for most
  * formats, two for the column-sorted SCOO (main, finish) and for
  * CSR-Adaptive with long-row chunks.  Splits the event-pair kernel timing of

@@ -990,16 +990,16 @@ static int csr_vector_t(uint32_t lanes, uint32_t rows, const uint32_t *rp, const
                         const T *a, const T *x, T *y, hipStream_t s)
 {
   const uint32_t grid = ceil_div_u32((uint64_t)rows * lanes, 256);
+  const dim3 g(grid), b(256);
   switch (lanes) {
-  case 2: k_csr_vector<T, 2><<<grid, 256, 0, s>>>(rows, rp, ci, a, x, y); break;
-  case 4: k_csr_vector<T, 4><<<grid, 256, 0, s>>>(rows, rp, ci, a, x, y); break;
-  case 8: k_csr_vector<T, 8><<<grid, 256, 0, s>>>(rows, rp, ci, a, x, y); break;
-  case 16: k_csr_vector<T, 16><<<grid, 256, 0, s>>>(rows, rp, ci, a, x, y); break;
-  case 32: k_csr_vector<T, 32><<<grid, 256, 0, s>>>(rows, rp, ci, a, x, y); break;
-  case 64: k_csr_vector<T, 64><<<grid, 256, 0, s>>>(rows, rp, ci, a, x, y); break;
+  case 2: return launch_k(k_csr_vector<T, 2>, g, b, 0, s, rows, rp, ci, a, x, y);
+  case 4: return launch_k(k_csr_vector<T, 4>, g, b, 0, s, rows, rp, ci, a, x, y);
+  case 8: return launch_k(k_csr_vector<T, 8>, g, b, 0, s, rows, rp, ci, a, x, y);
+  case 16: return launch_k(k_csr_vector<T, 16>, g, b, 0, s, rows, rp, ci, a, x, y);
+  case 32: return launch_k(k_csr_vector<T, 32>, g, b, 0, s, rows, rp, ci, a, x, y);
+  case 64: return launch_k(k_csr_vector<T, 64>, g, b, 0, s, rows, rp, ci, a, x, y);
   default: set_error("lanes must be a power of two in [2, 64]"); return MFP_ERR_ARG;
   }
-  return launched();
 }
 
 int launch_csr_scalar(int dtype, uint32_t rows, const uint32_t *rp, const uint32_t *ci,
@@ -1008,10 +1008,10 @@ int launch_csr_scalar(int dtype, uint32_t rows, const uint32_t *rp, const uint32
   if (rows == 0) return MFP_OK;
   const uint32_t grid = ceil_div_u32(rows, 64);
   if (dtype == MFP_F32)
-    k_csr_scalar<float><<<grid, 64, 0, s>>>(rows, rp, ci, (const float *)a, (const float *)x, (float *)y);
-  else
-    k_csr_scalar<double><<<grid, 64, 0, s>>>(rows, rp, ci, (const double *)a, (const double *)x, (double *)y);
-  return launched();
+    return launch_k(k_csr_scalar<float>, dim3(grid), dim3(64), 0, s, rows, rp, ci, (const float *)a,
+                    (const float *)x, (float *)y);
+  return launch_k(k_csr_scalar<double>, dim3(grid), dim3(64), 0, s, rows, rp, ci, (const double *)a,
+                  (const double *)x, (double *)y);
 }
 
 int launch_csr_vector(int dtype, uint32_t lanes, uint32_t rows, const uint32_t *rp,
@@ -1121,10 +1121,10 @@ int launch_coo_add(int dtype, uint64_t n, const uint32_t *cr, const uint32_t *cc
   if (n == 0) return MFP_OK;
   const uint32_t grid = coo_grid(n);
   if (dtype == MFP_F32)
-    k_coo<float><<<grid, 256, 0, s>>>(n, cr, cc, (const float *)cv, (const float *)x, (float *)y);
-  else
-    k_coo<double><<<grid, 256, 0, s>>>(n, cr, cc, (const double *)cv, (const double *)x, (double *)y);
-  return launched();
+    return launch_k(k_coo<float>, dim3(grid), dim3(256), 0, s, n, cr, cc, (const float *)cv, (const float *)x,
+                    (float *)y);
+  return launch_k(k_coo<double>, dim3(grid), dim3(256), 0, s, n, cr, cc, (const double *)cv, (const double *)x,
+                  (double *)y);
 }
 
 int launch_hyb_atomic(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const void *ev,
@@ -1135,12 +1135,10 @@ int launch_hyb_atomic(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, 
   const uint32_t cb = ncoo ? coo_grid(ncoo) : 0u;
   if (eb + cb == 0) return MFP_OK;
   if (dtype == MFP_F32)
-    k_hyb_atomic<float><<<eb + cb, 256, 0, s>>>(rows, w, ec, (const float *)ev, ncoo, cr, cc,
-                                                (const float *)cv, (const float *)x, (float *)y, eb);
-  else
-    k_hyb_atomic<double><<<eb + cb, 256, 0, s>>>(rows, w, ec, (const double *)ev, ncoo, cr, cc,
-                                                 (const double *)cv, (const double *)x, (double *)y, eb);
-  return launched();
+    return launch_k(k_hyb_atomic<float>, dim3(eb + cb), dim3(256), 0, s, rows, w, ec, (const float *)ev, ncoo, cr,
+                    cc, (const float *)cv, (const float *)x, (float *)y, eb);
+  return launch_k(k_hyb_atomic<double>, dim3(eb + cb), dim3(256), 0, s, rows, w, ec, (const double *)ev, ncoo, cr,
+                  cc, (const double *)cv, (const double *)x, (double *)y, eb);
 }
 
 int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t threads,
@@ -1162,12 +1160,12 @@ int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t thre
     return MFP_ERR_ARG;
   }
   const uint32_t real = ceil_div_u32(rows, ss);
-  auto go = [&](auto kern, size_t bytes, auto tv) {
+  auto go = [&](auto kern, size_t bytes, auto tv) -> int {
     using TV = decltype(tv);
     if (bytes > 65536)
       MFP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    kern<<<real, threads, bytes, s>>>(rows, ss, L, cr, cc, (const TV *)cv, sp, (const TV *)x, (TV *)y);
-    return MFP_OK;
+    return launch_k(kern, dim3(real), dim3(threads), (uint32_t)bytes, s, rows, ss, L, cr, cc, (const TV *)cv, sp,
+                    (const TV *)x, (TV *)y);
   };
   if (dtype == MFP_F64)
     MFP_CHECK(go(k_scoo<double, double>, lds, 0.0));
@@ -1175,7 +1173,7 @@ int launch_scoo(int dtype, uint32_t rows, uint32_t ss, uint32_t L, uint32_t thre
     MFP_CHECK(go(k_scoo<float, double>, 2 * lds, 0.0f));
   else
     MFP_CHECK(go(k_scoo<float, float>, lds, 0.0f));
-  return launched();
+  return MFP_OK;
 }
 
 template <typename T, int DT>
