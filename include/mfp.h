@@ -193,9 +193,11 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream s);
  * max_kernels)) is kernel i's own duration where its launcher records the
  * kernel's start and end (hipExtLaunchKernelGGL events: every FP32 / FP64
  * format), else the time from the previous event to the end of kernel i
- * (16-bit value plans, and the zero fill before COO and the atomic hybrid); *n_kernels is the number of kernels (at most 8; 1
- * with an empty interval when nothing is launched).  A plan's SpMV is one kernel for most
- * formats, two for the column-sorted SCOO (main, finish) and for
+ * (16-bit value plans, and the zero fill of y that COO and the atomic
+ * hybrid launch first); *n_kernels is the number of kernels (at most 8; 1
+ * with an empty interval when nothing is launched).  A plan's SpMV is one
+ * kernel for most formats; two for COO and the atomic hybrid (fill, kernel),
+ * the 2-kernel hybrid (ELL, COO), the column-sorted SCOO (main, finish) and
  * CSR-Adaptive with long-row chunks.  Splits the event-pair kernel timing of
  * gpu_matrix_multiplier.cu:92-112 per kernel; not for graph capture, not
  * re-entrant on one host thread. */

@@ -1021,15 +1021,36 @@ __global__ void k_compact(uint32_t rows, const uint32_t *__restrict__ mark,
 
 __global__ void k_store1(uint32_t *p, uint32_t v) { *p = v; }
 
+// Device allocations freed on every return path unless released.
+struct device_temps {
+  std::vector<void *> held;
+  int alloc(void **p, size_t bytes)
+  {
+    const int st = dmalloc(p, bytes);
+    if (st == MFP_OK) held.push_back(*p);
+    return st;
+  }
+  void release(void *p)
+  {
+    for (void *&q : held)
+      if (q == p) q = nullptr;
+  }
+  ~device_temps()
+  {
+    for (void *q : held) dfree(q);
+  }
+};
+
 int row_blocks_device(uint32_t rows, const uint32_t *rp, uint32_t W, uint32_t **blocks_out,
                       uint32_t *count_out, hipStream_t s)
 {
   MFP_ARG(W > 0, "nnz_per_wg must be positive");
   const uint64_t n = (uint64_t)rows + 2; // nodes 0..rows, sentinel rows+1
+  device_temps tmp;
   uint32_t *J = nullptr, *J2 = nullptr, *mark = nullptr;
-  MFP_CHECK(dmalloc((void **)&J, n * 4));
-  MFP_CHECK(dmalloc((void **)&J2, n * 4));
-  MFP_CHECK(dmalloc((void **)&mark, n * 4));
+  MFP_CHECK(tmp.alloc((void **)&J, n * 4));
+  MFP_CHECK(tmp.alloc((void **)&J2, n * 4));
+  MFP_CHECK(tmp.alloc((void **)&mark, n * 4));
   const uint32_t g = ceil_div_u32(n, 256);
   k_close<<<g, 256, 0, s>>>(rows, rp, W, J);
   MFP_CHECK(fill_u32(mark, 0, n, s));
@@ -1044,14 +1065,12 @@ int row_blocks_device(uint32_t rows, const uint32_t *rp, uint32_t W, uint32_t **
   uint32_t count = 0;
   MFP_CHECK(exclusive_scan_u32(mark, J2, (uint64_t)rows + 1, s, &count));
   uint32_t *out = nullptr;
-  MFP_CHECK(dmalloc((void **)&out, ((uint64_t)count + 1) * 4));
+  MFP_CHECK(tmp.alloc((void **)&out, ((uint64_t)count + 1) * 4));
   k_compact<<<ceil_div_u32((uint64_t)rows + 1, 256), 256, 0, s>>>(rows, mark, J2, out);
   k_store1<<<1, 1, 0, s>>>(out + count, rows);
   MFP_HIP(hipGetLastError());
   MFP_HIP(hipStreamSynchronize(s));
-  dfree(J);
-  dfree(J2);
-  dfree(mark);
+  tmp.release(out);
   *blocks_out = out;
   *count_out = count;
   return MFP_OK;

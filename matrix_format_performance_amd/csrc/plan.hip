@@ -605,6 +605,7 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
     return launch_ell(dt, m->rows, p->ell_w, p->ell_col, p->ell_val, x, y, s);
   case MFP_COO:
     MFP_CHECK(fill_bytes_zero(y, (size_t)m->rows * dtype_size(dt), s));
+    mark_kernel_end(); // the zero fill of y is its own interval under mfp_plan_kernel_times
     return launch_coo_add(dt, p->ncoo, p->coo_row, p->coo_col, p->coo_val, x, y, s);
   case MFP_SCOO:
     if (p->colsort) return launch_scoo_cs(dt, m->rows, p->cs, p->scoo_threads, x, y, s);
@@ -615,6 +616,7 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
     return launch_coo_add(dt, p->ncoo, p->coo_row, p->coo_col, p->coo_val, x, y, s);
   case MFP_HYB_ATOMIC:
     MFP_CHECK(fill_bytes_zero(y, (size_t)m->rows * dtype_size(dt), s));
+    mark_kernel_end();
     return launch_hyb_atomic(dt, m->rows, p->ell_w, p->ell_col, p->ell_val, p->ncoo, p->coo_row,
                              p->coo_col, p->coo_val, x, y, s);
   }
