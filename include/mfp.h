@@ -179,7 +179,13 @@ enum {
    * of LDS per row) accumulate in LDS, and rows much longer than a slice's
    * mean share get slices of their own.  No reference counterpart (the
    * reference's SCOO keeps row order, scoo_matrix_class.cpp:102). */
-  MFP_TUNE_SCOO_COLSORT = 4
+  MFP_TUNE_SCOO_COLSORT = 4,
+  /* The plan gets no buffer of partial results of its own (column-sorted
+   * SCOO partial slots, CSR-Adaptive long-row chunk sums, chunk sums of the
+   * binned 16-bit CSR): such a plan runs through mfp_plan_spmv_noalloc, and
+   * mfp_plan_spmv / mfp_plan_spmm refuse it (MFP_ERR_STATE) when it has
+   * partial results. */
+  MFP_TUNE_STATIC_PARTIALS = 8
 };
 
 typedef struct mfp_plan_s *mfp_plan;
@@ -203,6 +209,20 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream s);
  * re-entrant on one host thread. */
 int mfp_plan_kernel_times(mfp_plan p, const void *x, void *y, mfp_stream s, float *ms, int max_kernels,
                           int *n_kernels);
+/* mfp_plan_spmv that allocates nothing and needs no buffer of the plan's
+ * own: the partial results its kernels write and read back (column-sorted
+ * SCOO partial slots, CSR-Adaptive long-row chunk sums, chunk sums of the
+ * binned 16-bit CSR) go to a static device array of
+ * MFP_NOALLOC_SCRATCH_BYTES per device instead (held by each device that
+ * runs it until the process ends).  A plan whose partial results need more
+ * gets MFP_ERR_BUDGET.  Every partial result is written before it is read
+ * within one call, so nothing in the array has to start at zero.  All calls
+ * on one device share the array: calls that overlap in time on one device
+ * (two streams, two host threads) race.  No counterpart in the reference;
+ * for callers that cannot allocate (HIP graph capture, tight serving loops)
+ * and for holding many plans created with MFP_TUNE_STATIC_PARTIALS. */
+#define MFP_NOALLOC_SCRATCH_BYTES (32u << 20)
+int mfp_plan_spmv_noalloc(mfp_plan p, const void *x, void *y, mfp_stream s);
 /* Y = A X for k right-hand sides (k in {1,2,4,8,16}); X is cols x k and Y
  * rows x k, row-major; X is 16-byte aligned when k values make a multiple
  * of 16 bytes.  k > 1 runs on CSR-vector plans (f32 / f64; each column is

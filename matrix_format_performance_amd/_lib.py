@@ -60,6 +60,8 @@ class tuning(C.Structure):
 MFP_TUNE_NO_OFFSET16 = 1
 MFP_TUNE_IS_DEFAULT = 2
 MFP_TUNE_SCOO_COLSORT = 4
+MFP_TUNE_STATIC_PARTIALS = 8
+MFP_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
 class candidate(C.Structure):
@@ -150,6 +152,7 @@ SIGNATURES = {
     "mfp_tuning_default": (i32, [vp, i32, P(tuning)]),
     "mfp_plan_create": (i32, [P(vp), vp, i32, P(tuning), vp]),
     "mfp_plan_spmv": (i32, [vp, vp, vp, vp]),
+    "mfp_plan_spmv_noalloc": (i32, [vp, vp, vp, vp]),
     "mfp_plan_kernel_times": (i32, [vp, vp, vp, vp, P(C.c_float), i32, P(i32)]),
     "mfp_plan_spmm": (i32, [vp, u32, vp, vp, vp]),
     "mfp_batch_create": (i32, [P(vp), P(vp), u32, u32, P(tuning), vp]),

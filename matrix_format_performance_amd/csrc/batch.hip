@@ -357,8 +357,13 @@ int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream sv)
 {
   MFP_ARG(p && X && Y, "null pointer");
   if (k == 1) return mfp_plan_spmv(p, X, Y, sv);
-  if (p->format == MFP_SCOO && p->colsort)
+  if (p->format == MFP_SCOO && p->colsort) {
+    if (p->cs.slots && !p->cs.partial) {
+      set_error("plan created with MFP_TUNE_STATIC_PARTIALS has no partial slots for SpMM");
+      return MFP_ERR_STATE;
+    }
     return launch_scoo_cs_spmm(p->dtype, k, p->m->rows, p->cs, p->scoo_threads, X, Y, as_stream(sv));
+  }
   MFP_ARG(p->format == MFP_CSR_VECTOR, "SpMM with k > 1 runs on CSR-vector and column-sorted SCOO plans");
   MFP_ARG(p->dtype == MFP_F32 || p->dtype == MFP_F64, "SpMM values must be f32 or f64");
   const mfp_matrix m = p->m;

@@ -685,7 +685,8 @@ static int cs_scatter_t(uint64_t nnz, const uint32_t *perm, const uint32_t *col,
 
 int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t *rp,
                          const uint32_t *ci, const void *val, uint32_t R, uint32_t long_threshold,
-                         uint64_t item_entries, scoo_cs_layout *out, hipStream_t s)
+                         uint64_t item_entries, scoo_cs_layout *out, hipStream_t s,
+                         bool own_partials)
 {
   MFP_ARG(R >= 1 && R <= 65536 && (R & (R - 1)) == 0,
           "column-sorted SCOO slice size must be a power of two <= 65536");
@@ -842,7 +843,7 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   out->kmax = 1;
   if (dtype == MFP_F32 || dtype == MFP_F64)
     while (out->kmax < 16 && (uint64_t)R * out->kmax * 2 * 8 <= 163840) out->kmax *= 2;
-  if (st == MFP_OK && slots) st = dmalloc(&out->partial, (uint64_t)slots * R * out->kmax * ps);
+  if (st == MFP_OK && slots && own_partials) st = dmalloc(&out->partial, (uint64_t)slots * R * out->kmax * ps);
   if (st == MFP_OK) st = dmalloc(&out->ev, std::max<uint64_t>(nnz, 1) * vs);
   if (st == MFP_OK) st = dmalloc((void **)&out->ec, std::max<uint64_t>(nnz, 1) * 4);
   if (st != MFP_OK) return done(st);
@@ -941,7 +942,8 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   }
   out->device_bytes = nnz * (vs + 4) + ((uint64_t)S + 1) * 4 +
                       (uint64_t)out->NI * 20 + (uint64_t)out->NF * 16 + (uint64_t)NL * 4 +
-                      (NL ? ((uint64_t)rows + 31) / 32 * 4 : 0) + (uint64_t)slots * R * out->kmax * ps;
+                      (NL ? ((uint64_t)rows + 31) / 32 * 4 : 0) +
+                      (own_partials ? (uint64_t)slots * R * out->kmax * ps : 0);
   return done(MFP_OK);
 }
 

@@ -59,8 +59,11 @@ struct adaptive_layout {
   uint32_t nchunks = 0;
   void *partials = nullptr;
 };
+// own_partials = false: no chunk-sum buffer (the plan runs with the static
+// scratch of mfp_plan_spmv_noalloc)
 int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t nnz_per_wg,
-                          uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s);
+                          uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s,
+                          bool own_partials = true);
 void free_adaptive_layout(adaptive_layout *l);
 
 int launch_spmm_vector(int dtype, uint32_t k, uint32_t lanes, uint32_t rows, const uint32_t *rp,

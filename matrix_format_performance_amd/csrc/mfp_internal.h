@@ -150,7 +150,8 @@ struct scoo_cs_layout {
 };
 int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t *rp,
                          const uint32_t *ci, const void *val, uint32_t R, uint32_t long_threshold,
-                         uint64_t item_entries, scoo_cs_layout *out, hipStream_t s);
+                         uint64_t item_entries, scoo_cs_layout *out, hipStream_t s,
+                         bool own_partials = true);
 void scoo_cs_free(scoo_cs_layout *l);
 // Binned CSR over 16-bit values (plans of format MFP_CSR_ADAPTIVE): row lists
 // per lane count G = 2 << b (list[bin_off[b] .. bin_off[b+1])), chunks
@@ -164,11 +165,15 @@ struct lp_binned_layout {
   float2 *partial = nullptr;
   uint64_t device_bytes = 0;
 };
-int lp_binned_build(uint32_t rows, const uint32_t *rp_dev, lp_binned_layout *out, hipStream_t s);
+int lp_binned_build(uint32_t rows, const uint32_t *rp_dev, lp_binned_layout *out, hipStream_t s,
+                    bool own_partials = true);
 void lp_binned_free(lp_binned_layout *l);
 int launch_lp_binned(int dt, const lp_binned_layout &l, const uint32_t *rp, const uint32_t *ci, const void *a,
                      const void *x, void *y, hipStream_t s);
 int fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+// looks up the current device's address of mfp_plan_spmv_noalloc's static
+// scratch (plan.hip), once per device
+int static_scratch_setup();
 int fill_bytes_zero(void *p, size_t bytes, hipStream_t s);
 int convert_values(int dtype_in, const void *in, int dtype_out, void *out, uint64_t n,
                    hipStream_t s);

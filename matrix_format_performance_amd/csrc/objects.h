@@ -38,5 +38,7 @@ struct mfp_plan_s {
   mfp::scoo_cs_layout cs;
   // binned CSR over 16-bit values
   mfp::lp_binned_layout lpb;
+  // MFP_TUNE_STATIC_PARTIALS: no buffer of partial results of its own
+  bool static_partials = false;
 };
 

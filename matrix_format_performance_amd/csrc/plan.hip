@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace mfp {
@@ -43,7 +45,8 @@ void free_adaptive_layout(adaptive_layout *l)
 }
 
 int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t W,
-                          uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s)
+                          uint32_t long_row, uint32_t chunk, adaptive_layout *out, hipStream_t s,
+                          bool own_partials)
 {
   MFP_ARG(W >= 1 && adaptive_vectors_per_lane(dtype, W) <= 16, "nnz_per_wg too large");
   MFP_ARG(chunk >= 256, "chunk must be >= 256");
@@ -59,7 +62,7 @@ int build_adaptive_layout(int dtype, uint32_t rows, const uint32_t *rp, uint32_t
     st = exclusive_scan_u32(cnt, base, l.nblocks, s, &l.nchunks);
   }
   if (st == MFP_OK) st = dmalloc((void **)&l.chunks, (uint64_t)l.nchunks * sizeof(uint4));
-  if (st == MFP_OK) st = dmalloc(&l.partials, (uint64_t)l.nchunks * dtype_size(dtype));
+  if (st == MFP_OK && own_partials) st = dmalloc(&l.partials, (uint64_t)l.nchunks * dtype_size(dtype));
   if (st == MFP_OK && l.nchunks)
     k_chunk_fill<<<ceil_div_u32(l.nblocks, 256), 256, 0, s>>>(l.blocks, l.nblocks, cnt, base, l.chunks);
   if (st == MFP_OK && hipStreamSynchronize(s) != hipSuccess) st = MFP_ERR_HIP;
@@ -182,7 +185,7 @@ static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipS
     return MFP_OK;
   }
   if (p->format == MFP_CSR_ADAPTIVE) {
-    MFP_CHECK(lp_binned_build(m->rows, m->row_ptr, &p->lpb, s));
+    MFP_CHECK(lp_binned_build(m->rows, m->row_ptr, &p->lpb, s, !(t.flags & MFP_TUNE_STATIC_PARTIALS)));
     p->device_bytes = p->lpb.device_bytes;
     return MFP_OK;
   }
@@ -236,6 +239,37 @@ static int plan_create_lp(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipS
   p->device_bytes = need;
   return MFP_OK;
 }
+
+// ---- the allocation-free SpMV ------------------------------------------------
+// Partial results of mfp_plan_spmv_noalloc live in one static array per
+// device (each device that runs it holds MFP_NOALLOC_SCRATCH_BYTES until the
+// process ends).  In host code `scratch` names a host-side stand-in, not a
+// device address: the address comes from hipGetSymbolAddress, once per
+// device with that device current, and only that address reaches kernels.
+static __device__ __align__(256) unsigned char scratch[MFP_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[64] = {};
+
+namespace mfp {
+// set-up: called by the library's init for its device, and by the entry
+// point for a device that init never saw
+int static_scratch_setup()
+{
+  int dev = -1;
+  MFP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("device index beyond the static scratch table");
+    return MFP_ERR_STATE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    MFP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = ptr;
+  }
+  return MFP_OK;
+}
+} // namespace mfp
 
 extern "C" {
 
@@ -435,7 +469,7 @@ static int plan_create_scoo_cs(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t,
   p->colsort = true;
   p->scoo_threads = t.scoo_threads ? t.scoo_threads : 512;
   MFP_CHECK(scoo_cs_build_device(m->dtype, m->rows, m->cols, m->nnz, m->row_ptr, m->columns, m->values, R, T, item,
-                                 &p->cs, s));
+                                 &p->cs, s, !(t.flags & MFP_TUNE_STATIC_PARTIALS)));
   p->ss = R;
   p->slices = p->cs.S;
   p->L = p->cs.NI;
@@ -459,6 +493,7 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
   p->dtype = m->dtype;
   p->m = m;
   p->t = t;
+  p->static_partials = (t.flags & MFP_TUNE_STATIC_PARTIALS) != 0;
   const size_t vs = dtype_size(m->dtype);
   int st = MFP_OK;
   auto fail = [&](int code) {
@@ -482,9 +517,10 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
     break;
   case MFP_CSR_ADAPTIVE:
     st = build_adaptive_layout(m->dtype, m->rows, m->row_ptr, t.nnz_per_wg, t.long_row, t.chunk,
-                               &p->ad, s);
+                               &p->ad, s, !(t.flags & MFP_TUNE_STATIC_PARTIALS));
     if (st != MFP_OK) return fail(st);
-    p->device_bytes = ((uint64_t)p->ad.nblocks + 1) * 4 + (uint64_t)p->ad.nchunks * (16 + vs);
+    p->device_bytes = ((uint64_t)p->ad.nblocks + 1) * 4 +
+                      (uint64_t)p->ad.nchunks * (16 + (p->ad.partials ? vs : 0));
     break;
   case MFP_ELL:
   case MFP_HYB:
@@ -567,18 +603,36 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
   return MFP_OK;
 }
 
-int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
+// Partial results a plan's SpMV writes and reads back: column-sorted SCOO
+// slots, CSR-Adaptive long-row chunk sums, chunk sums of the binned 16-bit
+// CSR.  Bytes for one SpMV (0 when the plan has none).
+static uint64_t plan_partial_bytes(const mfp_plan_s *p)
 {
-  MFP_ARG(p && x && y, "null pointer");
-  hipStream_t s = as_stream(sv);
+  if (p->colsort) return (uint64_t)p->cs.slots * p->cs.R * (p->dtype == MFP_F64 ? 8 : 4);
+  if (p->format == MFP_CSR_ADAPTIVE && is_lp(p->dtype)) return (uint64_t)p->lpb.nchunks * sizeof(float2);
+  if (p->format == MFP_CSR_ADAPTIVE) return (uint64_t)p->ad.nchunks * dtype_size(p->dtype);
+  return 0;
+}
+
+// y = A x; partials, when not null, replaces the plan's buffer of partial
+// results (plan_partial_bytes of it)
+static int plan_spmv(mfp_plan p, const void *x, void *y, hipStream_t s, void *partials)
+{
   const mfp_matrix m = p->m;
   const int dt = p->dtype;
   if (is_lp(dt)) {
     if (p->format == MFP_CSR_VECTOR)
       return launch_csr_vector_lp(dt, p->t.lanes, m->rows, m->row_ptr, m->columns, m->values, x, y, s);
-    if (p->format == MFP_CSR_ADAPTIVE)
-      return launch_lp_binned(dt, p->lpb, m->row_ptr, m->columns, m->values, x, y, s);
-    if (p->format == MFP_SCOO) return launch_scoo_cs(dt, m->rows, p->cs, p->scoo_threads, x, y, s);
+    if (p->format == MFP_CSR_ADAPTIVE) {
+      lp_binned_layout l = p->lpb;
+      if (partials) l.partial = static_cast<float2 *>(partials);
+      return launch_lp_binned(dt, l, m->row_ptr, m->columns, m->values, x, y, s);
+    }
+    if (p->format == MFP_SCOO) {
+      scoo_cs_layout l = p->cs;
+      if (partials) l.partial = partials;
+      return launch_scoo_cs(dt, m->rows, l, p->scoo_threads, x, y, s);
+    }
     return launch_ell_lp(dt, p->off16, p->lp_block, m->rows, p->ell_w, p->lp_cols, p->lp_base, p->ell_val, x, y, s);
   }
   switch (p->format) {
@@ -598,7 +652,7 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
     g.chunks = p->ad.chunks;
     g.nchunks = p->ad.nchunks;
     g.chunk = p->t.chunk;
-    g.partials = p->ad.partials;
+    g.partials = partials ? partials : p->ad.partials;
     return launch_csr_adaptive(dt, g, m->values, x, y, s);
   }
   case MFP_ELL:
@@ -608,7 +662,11 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
     mark_kernel_end(); // the zero fill of y is its own interval under mfp_plan_kernel_times
     return launch_coo_add(dt, p->ncoo, p->coo_row, p->coo_col, p->coo_val, x, y, s);
   case MFP_SCOO:
-    if (p->colsort) return launch_scoo_cs(dt, m->rows, p->cs, p->scoo_threads, x, y, s);
+    if (p->colsort) {
+      scoo_cs_layout l = p->cs;
+      if (partials) l.partial = partials;
+      return launch_scoo_cs(dt, m->rows, l, p->scoo_threads, x, y, s);
+    }
     return launch_scoo(dt, m->rows, p->ss, p->L, p->scoo_threads, p->coo_row, p->coo_col,
                        p->coo_val, p->sp, x, y, s);
   case MFP_HYB:
@@ -622,6 +680,44 @@ int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
   }
   set_error("unknown format");
   return MFP_ERR_ARG;
+}
+
+int mfp_plan_spmv(mfp_plan p, const void *x, void *y, mfp_stream sv)
+{
+  MFP_ARG(p && x && y, "null pointer");
+  if (p->static_partials && plan_partial_bytes(p)) {
+    set_error("plan created with MFP_TUNE_STATIC_PARTIALS has no buffer of partial results: "
+              "run it with mfp_plan_spmv_noalloc");
+    return MFP_ERR_STATE;
+  }
+  return plan_spmv(p, x, y, as_stream(sv), nullptr);
+}
+
+int mfp_plan_spmv_noalloc(mfp_plan p, const void *x, void *y, mfp_stream sv)
+{
+  MFP_ARG(p && x && y, "null pointer");
+  MFP_CHECK(ensure_init());
+  int dev = -1;
+  MFP_HIP(hipGetDevice(&dev));
+  MFP_ARG(dev >= 0 && dev < 64, "device index beyond the static scratch table");
+  void *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    ptr = g_scratch_ptr[dev];
+  }
+  if (!ptr) {
+    MFP_CHECK(static_scratch_setup());
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    ptr = g_scratch_ptr[dev];
+  }
+  const uint64_t need = plan_partial_bytes(p);
+  if (need > MFP_NOALLOC_SCRATCH_BYTES) {
+    set_error("mfp_plan_spmv_noalloc: the plan's partial results take " + std::to_string(need) +
+              " bytes, more than the static scratch of " + std::to_string(MFP_NOALLOC_SCRATCH_BYTES) +
+              " bytes; use mfp_plan_spmv");
+    return MFP_ERR_BUDGET;
+  }
+  return plan_spmv(p, x, y, as_stream(sv), ptr);
 }
 
 int mfp_plan_kernel_times(mfp_plan p, const void *x, void *y, mfp_stream sv, float *ms, int max_kernels,

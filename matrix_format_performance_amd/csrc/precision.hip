@@ -529,7 +529,8 @@ int launch_ell_lp(int dt, bool off16, bool block, uint32_t rows, uint32_t w, con
 // LP_LONG split into chunks of LP_CHUNK entries, one wave each.
 constexpr uint32_t LP_LONG = 4096, LP_CHUNK = 1024;
 
-int lp_binned_build(uint32_t rows, const uint32_t *rp_dev, lp_binned_layout *out, hipStream_t s)
+int lp_binned_build(uint32_t rows, const uint32_t *rp_dev, lp_binned_layout *out, hipStream_t s,
+                    bool own_partials)
 {
   *out = lp_binned_layout{};
   std::vector<uint32_t> rp((size_t)rows + 1);
@@ -562,7 +563,8 @@ int lp_binned_build(uint32_t rows, const uint32_t *rp_dev, lp_binned_layout *out
   int st = dmalloc((void **)&out->list, std::max<size_t>(list.size(), 1) * 4);
   if (st == MFP_OK) st = dmalloc((void **)&out->chunks, std::max<size_t>(chunks.size(), 1) * sizeof(uint4));
   if (st == MFP_OK) st = dmalloc((void **)&out->longs, std::max<size_t>(longs.size(), 1) * sizeof(uint4));
-  if (st == MFP_OK) st = dmalloc((void **)&out->partial, std::max<size_t>(chunks.size(), 1) * sizeof(float2));
+  if (st == MFP_OK && own_partials)
+    st = dmalloc((void **)&out->partial, std::max<size_t>(chunks.size(), 1) * sizeof(float2));
   if (st != MFP_OK) {
     lp_binned_free(out);
     return st;
@@ -573,7 +575,8 @@ int lp_binned_build(uint32_t rows, const uint32_t *rp_dev, lp_binned_layout *out
   if (!longs.empty())
     MFP_HIP(hipMemcpyAsync(out->longs, longs.data(), longs.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
   MFP_HIP(hipStreamSynchronize(s));
-  out->device_bytes = list.size() * 4 + (chunks.size() + longs.size()) * sizeof(uint4) + chunks.size() * 8;
+  out->device_bytes = list.size() * 4 + (chunks.size() + longs.size()) * sizeof(uint4) +
+                      (own_partials ? chunks.size() * 8 : 0);
   return MFP_OK;
 }
 

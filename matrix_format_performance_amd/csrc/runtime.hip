@@ -51,7 +51,8 @@ static int init_device(int device)
   MFP_HIP(hipMemGetInfo(&fr, &tot));
   g_props.free_mem = fr;
   g_props.total_mem = tot;
-  return MFP_OK;
+  // the device address of the static scratch of mfp_plan_spmv_noalloc
+  return static_scratch_setup();
 }
 
 int ensure_init()

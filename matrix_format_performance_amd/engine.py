@@ -248,6 +248,11 @@ class Plan:
     def spmv(self, x: DeviceArray, y: DeviceArray, stream=None) -> None:
         L.call("mfp_plan_spmv", self.handle, x.ptr, y.ptr, stream)
 
+    def spmv_noalloc(self, x: DeviceArray, y: DeviceArray, stream=None) -> None:
+        """mfp_plan_spmv_noalloc: partial results in the library's static
+        scratch (no allocation; overlapping calls on one device race)."""
+        L.call("mfp_plan_spmv_noalloc", self.handle, x.ptr, y.ptr, stream)
+
     def kernel_times(self, x: DeviceArray, y: DeviceArray, stream=None) -> list:
         """One SpMV with an event after each kernel it launches; returns the
         per-kernel times in ms (waits for the SpMV)."""
