@@ -7,19 +7,19 @@ import numpy as np
 
 import matrix_format_performance_amd as mfp
 
-LOG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out",
-                   "gpu_test_cases.log")
+# MFP_CASE_LOG=<path>: also append each case name to that file
+LOG = os.environ.get("MFP_CASE_LOG")
 
 
 def announce(msg: str) -> None:
-    """Name the case on stderr and in a log file before it touches the GPU."""
+    """Name the case on stderr (and in MFP_CASE_LOG) before it touches the GPU."""
     print(f"[case] {msg}", file=sys.__stderr__, flush=True)
-    try:
-        os.makedirs(os.path.dirname(LOG), exist_ok=True)
-        with open(LOG, "a") as f:
-            f.write(msg + "\n")
-    except OSError:
-        pass
+    if LOG:
+        try:
+            with open(LOG, "a") as f:
+                f.write(msg + "\n")
+        except OSError:
+            pass
 
 
 def _edge_rows():

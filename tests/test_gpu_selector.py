@@ -14,7 +14,8 @@ from oracle import selector as S
 
 pytestmark = pytest.mark.gpu
 
-OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+# MFP_RESULTS_DIR=<dir>: also write the evaluation summary there
+OUT = os.environ.get("MFP_RESULTS_DIR")
 
 
 @pytest.mark.parametrize("name", ["stencil2d_64", "stencil3d_20", "powerlaw_20k", "uniform_band", "edge_rows",
@@ -148,9 +149,10 @@ def test_selector_held_out_regret(gpu, tmp_path):
                "fixed_format_held_out_mean_regret": fixed_regret,
                "autotune_s_mean": float(np.mean([r["autotune_s"] for r in recs])),
                "feature_s_mean": float(np.mean([r["feature_s"] for r in recs]))}
-    os.makedirs(OUT, exist_ok=True)
-    with open(os.path.join(OUT, "selector_small_eval.json"), "w") as f:
-        json.dump(summary, f, indent=1)
+    if OUT:
+        os.makedirs(OUT, exist_ok=True)
+        with open(os.path.join(OUT, "selector_small_eval.json"), "w") as f:
+            json.dump(summary, f, indent=1)
     print(json.dumps(summary))
     assert np.isfinite(rep["mean_regret"])
     # what the network gains over the method without it (one fixed format)

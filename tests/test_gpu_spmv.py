@@ -314,21 +314,78 @@ def test_ell_budget_refusal(gpu):
     A.destroy()
 
 
-@pytest.mark.parametrize("fmt,tuning,kernels", [
-    ("csr_scalar", {}, 1),
-    ("scoo", {"flags": 4, "scoo_slice_size": 256, "scoo_pieces": 3}, 2),
-])
-def test_plan_kernel_times(gpu, fmt, tuning, kernels):
-    # one SpMV with an event after each kernel: one interval per kernel, and
-    # the same y as a plain SpMV
-    rp, ci, va, cols = case("powerlaw_20k", "float32")
-    announce(f"kernel times fmt={fmt} tuning={tuning}")
-    x = gpu.generate_vector(7, cols, np.float32)
+# (format, tuning, matrix, kernels launched, same summation order every run)
+KERNEL_TIME_CASES = [
+    ("csr_scalar", {}, "powerlaw_20k", 1, True),
+    ("csr_vector", {"lanes": 8}, "powerlaw_20k", 1, True),
+    ("csr_adaptive", {}, "powerlaw_20k", 2, True),                                   # long-row chunks + finish
+    ("csr_adaptive", {"long_row": 65536, "chunk": 2048}, "powerlaw_20k", 1, True),  # no chunks
+    ("ell", {}, "stencil3d_20", 1, True),
+    ("coo", {}, "powerlaw_20k", 2, False),                                           # zero fill of y, kernel
+    ("scoo", {}, "powerlaw_20k", 1, False),                                          # reference geometry
+    ("scoo", {"flags": 4, "scoo_slice_size": 256, "scoo_pieces": 3}, "powerlaw_20k", 2, False),
+    ("hyb", {"hyb_width": 4}, "powerlaw_20k", 2, False),                             # ELL, COO
+    ("hyb_atomic", {"hyb_width": 4}, "powerlaw_20k", 2, False),                      # zero fill of y, kernel
+]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("fmt,tuning,name,kernels,ordered", KERNEL_TIME_CASES)
+def test_plan_kernel_times(gpu, fmt, tuning, name, kernels, ordered, dtype):
+    # one SpMV through the timed launch route (an event pair per kernel): one
+    # positive interval per kernel, and the same y as the plain launch route
+    rp, ci, va, cols = case(name, dtype)
+    announce(f"kernel times fmt={fmt} tuning={tuning} case={name} dtype={dtype}")
+    x = gpu.generate_vector(7, cols, np.dtype(dtype))
     A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
     plan = gpu.Plan(A, fmt, **tuning)
     xd = gpu.DeviceArray.from_host(x)
-    yd = gpu.DeviceArray(rp.size - 1, np.float32)
-    yd.fill(np.nan)
-    ms = plan.kernel_times(xd, yd)
-    assert len(ms) == kernels and all(t > 0 for t in ms)
-    _tol_check(yd.to_host(), rp, ci, va, x)
+    yt = gpu.DeviceArray(rp.size - 1, np.dtype(dtype))
+    yp = gpu.DeviceArray(rp.size - 1, np.dtype(dtype))
+    yt.fill(np.nan)
+    yp.fill(np.nan)
+    ms = plan.kernel_times(xd, yt)
+    plan.spmv(xd, yp)
+    gpu.synchronize()
+    assert len(ms) == kernels and all(t > 0 for t in ms), ms
+    y_timed, y_plain = yt.to_host(), yp.to_host()
+    if ordered:
+        assert np.array_equal(y_timed, y_plain)
+    else:
+        _, absdot = oracle.spmv_f64(rp, ci, va, x)
+        rtol = RTOL[np.dtype(dtype)]
+        assert oracle.tolerance_check(y_timed, y_plain, absdot, rtol) <= rtol
+    _tol_check(y_timed, rp, ci, va, x)
+    plan.destroy()
+    A.destroy()
+
+
+def test_plan_kernel_times_bf16_binned(gpu):
+    # the binned 16-bit CSR launches one kernel per non-empty length bin, the
+    # long-row chunks and their finish: an event after each, at most eight
+    rp, ci, va, cols = case("powerlaw_20k", "float32")
+    announce("kernel times bf16 binned csr_adaptive powerlaw_20k")
+    x = gpu.generate_vector(7, cols, np.float32)
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    B = A.convert("bf16")
+    plan = gpu.Plan(B, "csr_adaptive")
+    xd = gpu.DeviceArray.from_host(x)
+    yt = gpu.DeviceArray(rp.size - 1, np.float32)
+    yp = gpu.DeviceArray(rp.size - 1, np.float32)
+    yt.fill(np.nan)
+    ms = plan.kernel_times(xd, yt)
+    plan.spmv(xd, yp)
+    gpu.synchronize()
+    # expected: a kernel per non-empty bin of row lengths (<= 2, 4, ..., 32,
+    # <= 4096 entries) plus chunks and finish when a row exceeds 4096
+    lens = np.diff(rp.astype(np.int64))
+    short = lens[lens <= 4096]
+    bins = {min(5, max(0, int(np.ceil(np.log2(max(n, 1)))) - 1)) for n in short}
+    want = len(bins) + (2 if (lens > 4096).any() else 0)
+    assert len(ms) == want and all(t > 0 for t in ms), (ms, want)
+    y_timed = yt.to_host()
+    assert not np.isnan(y_timed).any()
+    assert np.array_equal(y_timed, yp.to_host())
+    plan.destroy()
+    B.destroy()
+    A.destroy()
