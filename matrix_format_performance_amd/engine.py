@@ -495,6 +495,10 @@ def batch_workload(count: int = 512, seed: int = 0) -> list:
 def workload(name: str, dtype=np.float32, seed: int = 0):
     if name == "stencil2d_256":
         return generate(GenSpec("stencil2d", values="laplace", n=256, seed=seed), dtype)
+    if name == "powerlaw_1m_normal":
+        # the precision study's power-law matrix: the same structure, values N(0, 1)
+        return generate(GenSpec("powerlaw", values="normal", rows=1 << 20, nnz_lo=1, nnz_hi=50000,
+                                alpha=2.0, seed=seed), dtype)
     if name == "powerlaw_1m":
         return generate(GenSpec("powerlaw", values="uniform", rows=1 << 20, nnz_lo=1, nnz_hi=50000,
                                 alpha=2.0, seed=seed), dtype)

@@ -47,6 +47,11 @@ def case(name: str, dtype_str: str):
         rp, ci, va = mfp.generate(mfp.GenSpec("powerlaw", values="uniform", rows=20000, nnz_lo=1,
                                               nnz_hi=12000, seed=3), dtype)
         cols = 20000
+    elif name == "powerlaw_narrow":
+        # power-law row lengths capped at 300: an ELL of it stays small
+        rp, ci, va = mfp.generate(mfp.GenSpec("powerlaw", values="uniform", rows=20000, nnz_lo=1,
+                                              nnz_hi=300, seed=3), dtype)
+        cols = 20000
     elif name == "uniform_band":
         rp, ci, va = mfp.generate(mfp.GenSpec("uniform", values="uniform", rows=30000, nnz_lo=4,
                                               nnz_hi=64, bandwidth=0.02, seed=4), dtype)

@@ -11,6 +11,9 @@ too.  On top of that, size-independent properties:
   per-row tolerance of the sequential CPU answer, linearity of the
   column-sorted SCOO, the FP64 GPU oracle and residual, SpMM with 4 and 16
   right-hand sides, and the ELL budget refusal (width about 50,000);
+- the 2D Laplacian again with x ~ U[-1, 1) from seed 0, per row against
+  the oracle for CSR-scalar, CSR-vector, ELL and COO; the power-law
+  matrix with N(0, 1) values stored as BF16 / FP16;
 - 3D 27-point 128^3 (2,097,152 rows, 55.7M nnz): ELL and CSR-vector
   bit-exact, the column-sorted SCOO within tolerance, x = 1 giving the row
   sums, and the 16-bit value forms against FP64 of the stored values;
@@ -298,4 +301,75 @@ def test_stencil3d_128_16bit_values(gpu, stencil3d, vt):
         assert not np.isnan(y).any()
         err = oracle.tolerance_check(y, y64, ad, 1.0)
         assert err <= bound, (fmt, err)
+    B.destroy()
+
+
+C1_FORMATS = [("csr_scalar", {}), ("csr_vector", {"lanes": 4}), ("ell", {}), ("coo", {})]
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("fmt,tuning", C1_FORMATS)
+def test_laplace_256_random_x_per_row(gpu, fmt, tuning, dtype):
+    # the small plain-method configuration as stated: x ~ U[-1, 1) from seed
+    # 0, CSR-scalar, CSR-vector, ELL and COO compared row by row with the
+    # oracle (bit for bit in the orders the first three fix; COO, whose
+    # atomics add in varying order, within the per-row tolerance)
+    rp, ci, va = _workload("stencil2d_256", dtype)
+    rows = rp.size - 1
+    announce(f"full size stencil2d_256 x=U(seed 0) fmt={fmt} dtype={dtype}")
+    x = gpu.generate_vector(0, rows, np.dtype(dtype))
+    assert x.min() >= -1 and x.max() < 1
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, rows)
+    y, info = _spmv(gpu, A, fmt, x, rows, **tuning)
+    A.destroy()
+    if fmt == "csr_scalar":
+        assert np.array_equal(y, oracle.spmv_seq(rp, ci, va, x))
+    elif fmt == "csr_vector":
+        assert np.array_equal(y, oracle.spmv_lanes(rp, ci, va, x, tuning["lanes"]))
+    elif fmt == "ell":
+        w, ed, ec = oracle.ell_from_csr(rp, ci, va)
+        assert info["ell_width"] == w == 5
+        assert np.array_equal(y, oracle.spmv_ell(rows, w, ec, ed, x))
+    _tol(y, rp, ci, va, x)
+
+
+@pytest.fixture(scope="module")
+def powerlaw_normal(gpu):
+    rp, ci, va = _workload("powerlaw_1m_normal", "float32")
+    rows = rp.size - 1
+    announce(f"full size powerlaw_1m N(0,1) values nnz={ci.size}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, rows)
+    yield rp, ci, va, A
+    A.destroy()
+
+
+@pytest.mark.parametrize("vt", ["bf16", "f16"])
+def test_powerlaw_1m_normal_16bit_values(gpu, powerlaw_normal, vt):
+    # the precision configuration's power-law matrix (values N(0, 1)) stored
+    # as BF16 / FP16: the column-sorted SCOO (exact products added in double),
+    # the binned CSR (Dot2-compensated FP32) and CSR-vector against FP64 of
+    # the stored values; the total error against FP64 of the FP32 values is
+    # the quantisation (about 2^-9 / sqrt(3) relative for BF16, 2^-11 / sqrt(3) for FP16)
+    rp, ci, va, A = powerlaw_normal
+    rows = rp.size - 1
+    announce(f"full size powerlaw_1m_normal {vt}")
+    B = A.convert(vt)
+    _, _, bits = B.download()
+    if vt == "bf16":
+        from oracle.selector import bf16_round
+        stored = bf16_round(va)
+        assert np.array_equal(bits, (stored.view(np.uint32) >> 16).astype(np.uint16))
+    else:
+        assert np.array_equal(bits, va.astype(np.float16).view(np.uint16))
+        stored = va.astype(np.float16).astype(np.float32)
+    x = gpu.generate_vector(4, rows, np.float32)
+    y64, ad = oracle.spmv_f64(rp, ci, stored, x)
+    y_true, _ = oracle.spmv_f64(rp, ci, va, x)
+    for fmt, kw, bound in (("scoo", COLSORT, 1e-6), ("csr_adaptive", {}, 2.5e-7), ("csr_vector", {"lanes": 8}, 2.5e-7)):
+        y, _ = _spmv(gpu, B, fmt, x, rows, **kw)
+        assert not np.isnan(y).any()
+        err = oracle.tolerance_check(y, y64, ad, 1.0)
+        assert err <= bound, (fmt, err)
+        rel_l2 = float(np.linalg.norm(y - y_true) / np.linalg.norm(y_true))
+        assert rel_l2 < (4e-3 if vt == "bf16" else 6e-4), (fmt, rel_l2)
     B.destroy()
