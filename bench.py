@@ -16,16 +16,26 @@ the steps), reported in config.format / config.tuning, with the best
 candidate of every format in the "formats" table.
 
 roofline: the same flushed steps again, timed per kernel of the SpMV
-(mfp_plan_kernel_times: the column-sorted SCOO's kernels record their own
-start and end, others an event after each kernel); the dominant kernel is
-the one with the longest mean time.  achieved = its algorithmic bytes / that mean, where a
-lone kernel is credited with B_alg and, in a two-kernel SpMV (a main kernel
-then one that adds partial slots or chunk sums into y), the dominant kernel
-with B_alg - rows*s_y (the matrix and x reads).  traffic = its HBM bytes
-per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/traffic_*.json).
+(mfp_plan_kernel_times: every FP32 / FP64 kernel records its own start and
+end); the dominant kernel is the one with the longest mean time.  achieved =
+its share of B_alg (kernel_bytes: all of it for a lone kernel; the matrix and
+x reads for the main kernel of the column-sorted SCOO, whose finish writes
+y; ...) / that mean.  step_achieved / step_frac are B_alg over the whole
+step's time (value).  traffic = the dominant kernel's HBM bytes per launch
+from rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/traffic_*.json).
+
+precision: best FP32 form against BF16 / FP16 value storage on the 3D
+27-point stencil and the power-law matrix with N(0, 1) values, with errors
+against FP64; GB/s there are effective rates (B_alg / time).  f64: the
+power-law workload in FP64 (fastest candidate, same step protocol).
+
+learned_pick: the shipped selector's pick on this matrix against the
+exhaustive best, and its held-out regret over the committed sweep.
 
 cpu_baseline: the oracle's restatement of the reference CPU path
-(cpu_matrix_multiplier.cpp), timed on this host on the same matrix.
+(cpu_matrix_multiplier.cpp), timed on this host on the same matrix with the
+affinity mask's threads (the analog of hardware_concurrency()), capped by
+OMP_NUM_THREADS; the CPU model and counts are recorded.
 """
 from __future__ import annotations
 
@@ -193,6 +203,27 @@ def kernel_names(fmt, info):
             "hyb_atomic": ["k_hyb_atomic"]}.get(fmt)
 
 
+def kernel_bytes(fmt, info, nbytes, rows, s_y, nk):
+    """Algorithmic bytes credited to each kernel of one SpMV, and how exact
+    that split is.  B_alg = matrix + x reads + y writes."""
+    y_bytes = rows * s_y
+    if nk == 1:
+        return [nbytes], "exact: one kernel does the whole SpMV"
+    if fmt == "scoo" and info["scoo_colsort"] and nk == 2:
+        return [nbytes - y_bytes, y_bytes], ("main: matrix and x reads; finish: the y writes (a slice with one "
+                                             "work item writes its rows of y in the main kernel)")
+    if fmt == "csr_adaptive" and nk == 2:
+        return [nbytes, 0], "main: everything but the y entries of the chunked long rows, which the finish writes"
+    if fmt in ("coo", "hyb_atomic") and nk == 2:
+        return [y_bytes, nbytes - y_bytes], "zero fill of y, then the kernel's matrix and x reads"
+    if fmt == "hyb" and nk == 2:
+        ell_share = 1.0 - info["coo_elements"] / max(info["nnz"], 1)
+        m = nbytes - y_bytes - (rows + 1) * 4 - info["cols"] * s_y
+        ell = int(ell_share * m) + (rows + 1) * 4 + info["cols"] * s_y + y_bytes
+        return [ell, nbytes - ell], "approximate: the matrix bytes split by the ELL / COO entry counts"
+    return [nbytes / nk] * nk, "approximate: B_alg split evenly over the kernels"
+
+
 def time_plan(mfp, plan, xd, yd, reps, flush, timer):
     ms = []
     for _ in range(reps):
@@ -222,7 +253,28 @@ def measured_copy_gbs(mfp, timer, nbytes=1 << 30, reps=5):
     return 2 * nbytes / (statistics.median(ms[1:]) * 1e-3) / 1e9
 
 
-def cpu_baseline(rp, ci, va, x, nbytes, threads):
+def host_cpu():
+    """CPU model, nproc and the threads the CPU baseline uses: the affinity
+    mask's size (the analog of std::thread::hardware_concurrency(),
+    cpu_matrix_multiplier.cpp:263), capped by OMP_NUM_THREADS where the host
+    grants this process that many threads."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else affinity
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads": int(cap) if cap and cap.isdigit() else None, "threads": threads}
+
+
+def cpu_baseline(rp, ci, va, x, nbytes, threads, cpu):
     import oracle
     _, t1 = oracle.spmv_seq_timed(rp, ci, va, x, reps=3)
     times = []
@@ -236,6 +288,7 @@ def cpu_baseline(rp, ci, va, x, nbytes, threads):
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
+        "cpu": cpu,
         "sample": (f"full matrix, oracle restatement of cpu_csr_spmv_multi_thread_naive "
                    f"(cpu_matrix_multiplier.cpp:229-308), {threads} threads, median of {len(times)} "
                    f"passes = {tm * 1e3:.3f} ms; single-thread cpu_csr_spmv_single_thread_naive "
@@ -251,7 +304,7 @@ def precision_study(mfp, timer, flush, reps, cache):
     against the GPU FP64 oracle of the stored values (arithmetic) and of the
     FP32 values (arithmetic + quantisation)."""
     out = {}
-    for wl in ("stencil3d_128", "powerlaw_1m"):
+    for wl in ("stencil3d_128", "powerlaw_1m_normal"):
         rp, ci, va = cache[wl] if wl in cache else mfp.workload(wl, np.float32)
         rows, nnz = rp.size - 1, ci.size
         A = mfp.DeviceMatrix.from_csr(rp, ci, va, rows)
@@ -292,10 +345,17 @@ def precision_study(mfp, timer, flush, reps, cache):
                 ys.free()
                 ads.free()
             p.destroy()
-            res[vt] = {"format": fmt, "ms": round(ms, 5), "GB_per_s": round(nb / (ms * 1e-3) / 1e9, 1),
+            res[vt] = {"format": fmt, "ms": round(ms, 5),
+                       "effective_GB_per_s": round(nb / (ms * 1e-3) / 1e9, 1),
                        "algorithmic_bytes": nb, "rel_l2_vs_fp64_stored": arith["rel_l2"],
                        "max_row_err_vs_fp64_stored": arith["max_row_err"],
                        "rel_l2_vs_fp64_fp32_values": total["rel_l2"]}
+            tpath = os.path.join(ROOT, "profiles", f"traffic_{wl}_{vt}_{fmt}.json")
+            if os.path.exists(tpath):
+                with open(tpath) as f:
+                    hb = json.load(f)["hbm_bytes_per_launch"]
+                res[vt]["counter_hbm_bytes"] = hb
+                res[vt]["counter_GB_per_s"] = round(hb / (ms * 1e-3) / 1e9, 1)
             if vt != "f32":
                 res[vt]["speedup_vs_f32"] = round(res["f32"]["ms"] / ms, 3)
                 M.destroy()
@@ -305,7 +365,69 @@ def precision_study(mfp, timer, flush, reps, cache):
             a.free()
         A.destroy()
         out[wl] = res
+    out["note"] = ("effective_GB_per_s = B_alg / time with B_alg = nnz (s_v + 4) + (rows + 1) 4 + x + y; the "
+                   "16-bit sliced ELL stores 2-byte column offsets beside its values, so its effective rate can "
+                   "exceed the HBM rate; counter_GB_per_s (where profiles/ holds the counters) is the HBM traffic "
+                   "the FETCH_SIZE / WRITE_SIZE counters measured per launch over the same time")
     return out
+
+
+def f64_leg(mfp, timer, flush, reps):
+    """The irregular workload in FP64: the fastest autotuner candidate,
+    timed like the headline steps."""
+    rp, ci, va = mfp.workload("powerlaw_1m", np.float64)
+    rows, nnz = rp.size - 1, ci.size
+    A = mfp.DeviceMatrix.from_csr(rp, ci, va, rows)
+    xd = mfp.DeviceArray.from_host(mfp.generate_vector(1, rows, np.float64))
+    yd = mfp.DeviceArray(rows, np.float64)
+    cands = mfp.autotune(A, xd, yd, reps=reps, flush=flush)
+    ok = [c for c in cands if c["status"] == 0 and np.isfinite(c["ms"])]
+    best = ok[0]
+    p = mfp.Plan(A, best["format"], **best["tuning"])
+    time_plan(mfp, p, xd, yd, 1, flush, timer)
+    ms = statistics.fmean(time_plan(mfp, p, xd, yd, 20, flush, timer))
+    y64, ad = A.oracle_f64(xd)
+    p.spmv(xd, yd)
+    err = mfp.residual(yd, y64, ad)["max_row_err"]
+    nb = b_alg(rows, rows, nnz, 8, 8)
+    out = {"workload": "powerlaw_1m", "values": "f64", "format": best["format"],
+           "tuning": tuning_summary(best["format"], best["tuning"]), "ms": round(ms, 5),
+           "GB_per_s": round(nb / (ms * 1e-3) / 1e9, 1), "frac_hbm": round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "algorithmic_bytes": nb, "max_row_err_vs_fp64_gpu_oracle": err, "candidates": len(cands)}
+    log(f"[bench] f64 powerlaw_1m: {best['format']} {ms * 1e3:.1f} us {out['GB_per_s']} GB/s")
+    for o in (p, A):
+        o.destroy()
+    for a in (xd, yd, y64, ad):
+        a.free()
+    return out
+
+
+def held_out_regret(mfp):
+    """The shipped selector on the held-out fifth (480) of the committed
+    2,400-matrix sweep, BF16 inference on the matrix cores: the regret of
+    its picks against each matrix's exhaustive best."""
+    from matrix_format_performance_amd import selector as S
+    import gzip
+    recs = []
+    for name in ("selector_c5_sweep.jsonl.gz", "selector_c5_sweep_big.jsonl.gz"):
+        path = os.path.join(ROOT, "results", name)
+        if not os.path.exists(path):
+            return None
+        byidx = {}
+        with gzip.open(path, "rt") as f:
+            for line in f:
+                r = json.loads(line)
+                byidx[r["index"]] = r
+        recs += [byidx[k] for k in sorted(byidx)]
+    te = [r for i, r in enumerate(recs) if i % 5 == 4]
+    sl = S.Selector.default()
+    probs = sl.net.predict(np.array([r["features"] for r in te], np.float32), bf16=True)
+    ev = S.evaluate_with_rules(te, probs, sl.rules)
+    sl.net.destroy()
+    ev = {k: round(v, 4) for k, v in ev.items()}
+    ev.update({"matrices": len(te), "source": "results/selector_c5_sweep*.jsonl.gz, every fifth record held out; "
+               "BF16 inference on the matrix cores, tuning by the rules"})
+    return ev
 
 
 def main():
@@ -318,7 +440,8 @@ def main():
     ap.add_argument("--format", default="auto")
     ap.add_argument("--select-reps", type=int, default=5)
     ap.add_argument("--no-flush", action="store_true", help="warm-cache timing (labelled)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0 = the affinity mask, capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 / FP16 study")
@@ -424,6 +547,10 @@ def main():
         sl.net.destroy()
         log(f"[bench] learned pick {pfmt} {selector_out['tuning']}: {pms * 1e3:.1f} us "
             f"({selector_out['slowdown_vs_exhaustive']}x the exhaustive best)")
+        selector_out["held_out"] = held_out_regret(mfp)
+        if selector_out["held_out"]:
+            log(f"[bench] selector on {selector_out['held_out']['matrices']} held-out matrices: mean regret "
+                f"{selector_out['held_out']['mean_regret']}, p90 {selector_out['held_out']['p90_regret']}")
 
     # correctness of the measured format on this very workload
     time_plan(mfp, plan, xd, yd, 1, flush, timer)
@@ -460,12 +587,13 @@ def main():
     nk = min(len(k) for k in per)
     kernel_ms = [statistics.fmean(k[i] for k in per) for i in range(nk)]
     dom = max(range(nk), key=lambda i: kernel_ms[i])
-    # a lone kernel does the whole SpMV; where a second kernel writes y
-    # (partial slots or chunk sums added), the dominant one is credited with
-    # the compulsory reads only (matrix and x)
-    y_bytes = rows * s
-    dom_bytes = nbytes if nk == 1 else nbytes - y_bytes
-    dom_names = None if args.values else kernel_names(best, plan.info())
+    # each kernel is credited with its share of B_alg (kernel_bytes)
+    pinfo = plan.info()
+    shares, share_note = kernel_bytes(best, pinfo, nbytes, rows, s, nk)
+    if args.values:
+        share_note = "approximate: 16-bit value plans time each kernel from the previous event; " + share_note
+    dom_bytes = shares[dom]
+    dom_names = None if args.values else kernel_names(best, pinfo)
     achieved = dom_bytes / (kernel_ms[dom] * 1e-3) / 1e9
 
     traffic = None
@@ -521,8 +649,12 @@ def main():
             "traffic": traffic,
             "kernel": dom_names[dom] if dom_names and len(dom_names) == nk else f"kernel {dom + 1} of {nk}",
             "ms_per_launch": round(kernel_ms[dom], 5),
-            "algorithmic_bytes_per_launch": dom_bytes,
+            "algorithmic_bytes_per_launch": int(dom_bytes),
             "kernel_ms": [round(t, 5) for t in kernel_ms],
+            "kernel_bytes": [int(b) for b in shares],
+            "bytes_split": share_note,
+            "step_achieved": round(value, 2),
+            "step_frac": round(value / HBM_PEAK_GBS, 4),
         },
     }
     # host-to-host rate: upload CSR and x, build the plan, one SpMV, download y
@@ -547,12 +679,15 @@ def main():
     plan.destroy()
 
     if args.dtype == "f32" and not args.values and not args.no_precision:
-        cache = {args.workload: (rp, ci, va)} if args.bandwidth >= 1.0 else {}
         A.destroy()
-        out["precision"] = precision_study(mfp, timer, flush, args.select_reps, cache)
+        out["precision"] = precision_study(mfp, timer, flush, args.select_reps, {})
+        if args.workload == "powerlaw_1m" and args.bandwidth >= 1.0:
+            out["f64"] = f64_leg(mfp, timer, flush, args.select_reps)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(rp, ci, va, x, nbytes, args.cpu_threads)
+        cpu = host_cpu()
+        threads = args.cpu_threads or cpu["threads"]
+        out["cpu_baseline"] = cpu_baseline(rp, ci, va, x, nbytes, threads, cpu)
     print(json.dumps(out), flush=True)
 
 
