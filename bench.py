@@ -224,6 +224,51 @@ def kernel_bytes(fmt, info, nbytes, rows, s_y, nk):
     return [nbytes / nk] * nk, "approximate: B_alg split evenly over the kernels"
 
 
+def rocsparse_row(mfp, A, xd, yd, rp, ci, va, x, nbytes, reps, flush, timer):
+    """The "GPU CSR (rocSPARSE)" comparator (the reference's cuSPARSE slot,
+    gpu_matrix_multiplier.cu:135-257): rocsparse_spmv's CSR-adaptive
+    algorithm on the same device CSR, timed like the candidates (cache
+    flush, event pair, median).  Benchmark-only: librocsparse.so is opened at
+    run time by app/rocsparse_slot.cpp; None without it."""
+    import ctypes as C
+    path = os.path.join(ROOT, "matrix_format_performance_amd", "librocsparse_slot.so")
+    if not os.path.exists(path) or A.value_type not in ("f32", "f64"):
+        return None
+    lib = C.CDLL(path)
+    lib.mfp_rs_error.restype = C.c_char_p
+    if not lib.mfp_rs_available():
+        return {"skipped": lib.mfp_rs_error().decode()}
+    lib.mfp_rs_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.mfp_rs_plan_spmv.argtypes = [C.c_void_p, C.c_void_p]
+    lib.mfp_rs_plan_destroy.argtypes = [C.c_void_p]
+    p = C.c_void_p()
+    dt = mfp._lib.MFP_F32 if A.value_type == "f32" else mfp._lib.MFP_F64
+    if lib.mfp_rs_plan_create(C.byref(p), dt, A.rows, A.cols, A.nnz, A.row_ptr_ptr, A.columns_ptr, A.values_ptr,
+                              xd.ptr, yd.ptr, None) != 0:
+        return {"skipped": lib.mfp_rs_error().decode()}
+    ms = []
+    for _ in range(reps + 1):
+        if flush:
+            mfp.flush_cache(1 << 30)
+        timer.start()
+        lib.mfp_rs_plan_spmv(p, None)
+        timer.stop()
+        ms.append(timer.ms())
+    mfp.synchronize()
+    lib.mfp_rs_plan_destroy(p)
+    import oracle
+    y = yd.to_host()
+    _, absdot = oracle.spmv_f64(rp, ci, va, x)
+    err = oracle.tolerance_check(y, oracle.spmv_seq(rp, ci, va, x), absdot, 1.0)
+    med = statistics.median(ms[1:])
+    log(f"[bench] rocsparse      {med * 1e3:9.2f} us  {nbytes / (med * 1e-3) / 1e9:8.1f} GB/s  (comparator)")
+    return {"ms": round(med, 5), "GB_per_s": round(nbytes / (med * 1e-3) / 1e9, 1),
+            "frac_hbm": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "label": "GPU CSR (rocSPARSE)",
+            "tuning": {"algorithm": "rocsparse_spmv_alg_csr_adaptive"}, "max_norm_err_vs_cpu_reference": err,
+            "note": "benchmark-only comparator, never a selection candidate"}
+
+
 def time_plan(mfp, plan, xd, yd, reps, flush, timer):
     ms = []
     for _ in range(reps):
@@ -517,6 +562,9 @@ def main():
                     "frac_hbm": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "tuning": tuning_summary(f, c["tuning"])}
         log(f"[bench] {f:13s} {med * 1e3:9.2f} us  {table[f]['GB_per_s']:8.1f} GB/s  {table[f]['tuning']}")
+    rs = rocsparse_row(mfp, A, xd, yd, rp, ci, va, x, nbytes, args.select_reps, flush, timer)
+    if rs is not None:
+        table["rocsparse"] = rs
     ok = [c for c in cands if c["status"] == 0 and np.isfinite(c["ms"])]
     best_c = ok[0]
     best = best_c["format"]

@@ -20,6 +20,8 @@
 //                    omit only the "GPU ELL" row instead of the whole matrix
 //                    (the reference skips the matrix, main.cpp:105-112)
 //   --no-cpu         omit the CPU rows (the reference answer is still computed)
+//   --no-rocsparse   omit the "GPU CSR (rocSPARSE)" row (left out anyway when
+//                    librocsparse.so cannot be opened)
 //
 // Synthetic names: stencil2d_256 (2D 5-point, 256 x 256), powerlaw_1m
 // (1,048,576 rows, power-law row lengths 1..50,000), stencil3d_128 (3D
@@ -27,6 +29,7 @@
 // stencil3d, powerlaw, uniform and keys n, rows, lo, hi, alpha, bw, seed,
 // values (laplace|uniform|normal).
 #include "../../include/mfp_compat.hpp"
+#include "rocsparse_slot.h"
 
 #include <charconv>
 #include <cstdio>
@@ -88,7 +91,7 @@ std::string get_filename(const std::string &path)
 struct options {
   std::string out = ".";
   unsigned reps = 20;
-  bool meta_only = false, ell_skip = false, cpu = true;
+  bool meta_only = false, ell_skip = false, cpu = true, rocsparse = true;
   std::vector<std::string> synthetic;
   std::string list;
 };
@@ -171,6 +174,49 @@ matrix_market::matrix_class generate(const std::string &name)
   return matrix_market::matrix_class(meta, std::move(ri), std::move(ci), std::move(va));
 }
 
+// ---- "GPU CSR (rocSPARSE)": the counterpart of gpu_csr_cusparse_spmv
+// (gpu/gpu_matrix_multiplier.cu:175-257), benchmark-only: upload, x = 1,
+// one timed rocsparse_spmv compute stage, y into reusable_vector, compare.
+// The byte model is the reference's cuSPARSE one (:245-249).
+template <typename T>
+measurement_class gpu_csr_rocsparse_spmv(const csr_matrix_class<T> &matrix, resizable_gpu_memory<T> &A,
+                                         resizable_gpu_memory<unsigned int> &col_ids,
+                                         resizable_gpu_memory<unsigned int> &row_ptr, resizable_gpu_memory<T> &x,
+                                         resizable_gpu_memory<T> &y, T *reusable_vector, const T *reference_y)
+{
+  const auto &meta = matrix.meta;
+  const size_t nnz = meta.non_zero_count, rows = meta.rows_count, cols = meta.cols_count;
+  A.resize(std::max<size_t>(nnz, 1));
+  col_ids.resize(std::max<size_t>(nnz, 1));
+  row_ptr.resize(rows + 1);
+  x.resize(std::max<size_t>(cols, 1));
+  y.resize(std::max<size_t>(rows, 1));
+  mfp_compat::check(mfp_memcpy_h2d(A.get(), matrix.data.get(), nnz * sizeof(T)), "upload values");
+  mfp_compat::check(mfp_memcpy_h2d(col_ids.get(), matrix.columns.get(), nnz * 4), "upload columns");
+  mfp_compat::check(mfp_memcpy_h2d(row_ptr.get(), matrix.row_ptr.get(), (rows + 1) * 4), "upload row_ptr");
+  const int dt = mfp_compat::dtype_of<T>();
+  mfp_compat::check(mfp_fill(dt, x.get(), cols, 1.0, nullptr), "x = 1");
+  mfp_rs_plan plan = nullptr;
+  if (mfp_rs_plan_create(&plan, dt, (uint32_t)rows, (uint32_t)cols, nnz, row_ptr.get(), col_ids.get(), A.get(),
+                         x.get(), y.get(), nullptr) != MFP_OK)
+    throw std::runtime_error(mfp_rs_error());
+  mfp_timer t = nullptr;
+  mfp_compat::check(mfp_timer_create(&t), "timer");
+  mfp_compat::check(mfp_synchronize(), "sync");
+  mfp_compat::check(mfp_timer_start(t, nullptr), "timer start");
+  const int st = mfp_rs_plan_spmv(plan, nullptr);
+  mfp_compat::check(mfp_timer_stop(t, nullptr), "timer stop");
+  float ms = 0;
+  mfp_compat::check(mfp_timer_elapsed_ms(t, &ms), "elapsed");
+  mfp_timer_destroy(t);
+  mfp_rs_plan_destroy(plan);
+  if (st != MFP_OK) throw std::runtime_error(mfp_rs_error());
+  mfp_compat::check(mfp_memcpy_d2h(reusable_vector, y.get(), rows * sizeof(T)), "download y");
+  mfp_compare_results(dt, (uint32_t)rows, reusable_vector, reference_y);
+  const double bytes = nnz * sizeof(T) + nnz * sizeof(T) + nnz * 4.0 + 2.0 * rows * 4 + rows * sizeof(T);
+  return measurement_class("GPU CSR (rocSPARSE)", ms / 1000.0, bytes, 2.0 * nnz);
+}
+
 // ---- one matrix in one precision (main.cpp:82-283) ----------------------------
 template <typename T>
 std::vector<measurement_class> perform_measurement(const matrix_market::matrix_class &mtx, const options &opt,
@@ -229,6 +275,12 @@ std::vector<measurement_class> perform_measurement(const matrix_market::matrix_c
   repeat([&](bool) {
     return gpu_csr_adaptive_spmv<T>(csr, A, col_ids, row_ptr, x_gpu, y, x.get(), reference.get());
   });
+  // the reference's cuSPARSE row comes here (main.cpp:194-197); its
+  // rocSPARSE counterpart runs when the library is present
+  if (opt.rocsparse && mfp_rs_available())
+    repeat([&](bool) {
+      return gpu_csr_rocsparse_spmv<T>(csr, A, col_ids, row_ptr, x_gpu, y, x.get(), reference.get());
+    });
   if (ell) repeat([&](bool) { return gpu_ell_spmv<T>(*ell, A, col_ids, x_gpu, y, x.get(), reference.get()); });
   repeat([&](bool) { return gpu_coo_spmv<T>(coo, A, col_ids, row_ptr, x_gpu, y, x.get(), reference.get()); });
   {
@@ -341,6 +393,7 @@ int main(int argc, char **argv)
       else if (a == "--meta-only") opt.meta_only = true;
       else if (a == "--ell-skip") opt.ell_skip = true;
       else if (a == "--no-cpu") opt.cpu = false;
+      else if (a == "--no-rocsparse") opt.rocsparse = false;
       else if (a == "--synthetic") {
         std::stringstream ss(next());
         std::string s;
@@ -353,7 +406,7 @@ int main(int argc, char **argv)
     }
   }
   if (opt.synthetic.empty() && opt.list.empty()) {
-    std::cerr << "Usage: " << argv[0] << " [--out DIR] [--reps N] [--meta-only] [--ell-skip] [--no-cpu] "
+    std::cerr << "Usage: " << argv[0] << " [--out DIR] [--reps N] [--meta-only] [--ell-skip] [--no-cpu] [--no-rocsparse] "
                  "(/path/to/mtx_list | --synthetic NAME[;NAME...])\n";
     return 1;
   }

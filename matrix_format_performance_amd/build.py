@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmfp.so")
 APP = os.path.join(HERE, "mfp_bench")
+RS_SLOT = os.path.join(HERE, "librocsparse_slot.so")
 
 SOURCES = [
     "runtime.hip",
@@ -83,13 +84,29 @@ def build_app() -> str:
     cxx = os.environ.get("CXX", "g++")
     tmp = APP + ".tmp"
     cmd = [cxx, "-O3", "-std=c++17", "-march=x86-64-v3", "-Wall", "-Wextra", "-Wno-unused-parameter",
-           os.path.join(HERE, "app", "mfp_bench.cpp"), "-o", tmp, "-L" + HERE, "-lmfp",
-           "-Wl,-rpath,$ORIGIN", "-lpthread"]
+           os.path.join(HERE, "app", "mfp_bench.cpp"), os.path.join(HERE, "app", "rocsparse_slot.cpp"), "-o", tmp,
+           "-L" + HERE, "-lmfp", "-Wl,-rpath,$ORIGIN", "-lpthread", "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"mfp_bench build failed:\n{r.stderr}")
     os.replace(tmp, APP)
+    build_rocsparse_slot()
     return APP
+
+
+def build_rocsparse_slot() -> str:
+    """The benchmark-only rocSPARSE comparator (app/rocsparse_slot.cpp) as a
+    shared library for bench.py; it opens librocsparse.so at run time."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = RS_SLOT + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+           os.path.join(HERE, "app", "rocsparse_slot.cpp"), "-o", tmp, "-L" + HERE, "-lmfp",
+           "-Wl,-rpath,$ORIGIN", "-ldl"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"rocSPARSE slot build failed:\n{r.stderr}")
+    os.replace(tmp, RS_SLOT)
+    return RS_SLOT
 
 
 if __name__ == "__main__":

@@ -22,6 +22,13 @@ FILES = ["matrices_info.json"] + [f"{p}{s}.json" for p in ("float", "double")
                                   for s in ("", "_effective_bandwidth", "_computational_throughput")]
 
 
+def _rocsparse_available():
+    """Whether the benchmark-only rocSPARSE slot can open librocsparse.so here."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(HERE, "matrix_format_performance_amd", "librocsparse_slot.so"))
+    return bool(lib.mfp_rs_available())
+
+
 def _run(args, out, timeout=300):
     announce("mfp_bench " + " ".join(args))
     r = subprocess.run([APP, "--out", str(out), *args], capture_output=True, text=True, timeout=timeout)
@@ -52,7 +59,10 @@ def test_synthetic_run_writes_the_seven_files(gpu, tmp_path):
             t = js[f"{prec}.json"][name]
             bw = js[f"{prec}_effective_bandwidth.json"][name]
             fl = js[f"{prec}_computational_throughput.json"][name]
-            assert set(t) == GPU_LABELS | {"CPU CSR", "CPU CSR Parallel"}
+            # the rocSPARSE row (the cuSPARSE slot of main.cpp:194-197) is there
+            # exactly when librocsparse.so can be opened
+            rs = {"GPU CSR (rocSPARSE)"} if _rocsparse_available() else set()
+            assert set(t) == GPU_LABELS | {"CPU CSR", "CPU CSR Parallel"} | rs
             assert set(bw) == set(t) == set(fl)
             assert all(v > 0 for v in t.values())
             csr_b = oracle.reference_bytes(0, dt, rows, nnz)
@@ -64,6 +74,8 @@ def test_synthetic_run_writes_the_seven_files(gpu, tmp_path):
                 "GPU CSR-Adaptive": (0.0, 0.0), "GPU SCOO": (0.0, 0.0), "GPU Hybrid (atomic)": (0.0, 0.0),
                 "GPU Hybrid (TODO)": (1.0, 1.0),
             }
+            if rs:  # the reference's cuSPARSE byte model (gpu_matrix_multiplier.cu:245-249)
+                models["GPU CSR (rocSPARSE)"] = (csr_b, 2.0 * nnz)
             for label, (b, ops) in models.items():
                 assert bw[label] == pytest.approx(b / (t[label] * 1e9), rel=1e-12), (prec, label)
                 assert fl[label] == pytest.approx(ops / (t[label] * 1e9), rel=1e-12), (prec, label)
