@@ -439,6 +439,16 @@ int mfp_gpu_hybrid_atomic_spmv(int dtype, const mfp_host_ell *ell, const mfp_hos
                                mfp_buffer coo_col_ids, mfp_buffer coo_row_ids, mfp_buffer x,
                                mfp_buffer y, void *reusable_vector, const void *reference_y,
                                mfp_measurement *out);
+/* gpu_hybrid_cpu_coo_spmv (gpu/gpu_matrix_multiplier.h:141-152, .cu:1111-1217):
+ * the ELL part on the GPU while the host sums the COO tail into cpu_y (rows
+ * entries, caller-owned, zeroed here); the tail is copied into tmp and added
+ * to y.  The timed span runs from the ELL launch to the end of the add, host
+ * loop included.  The reference's driver keeps this row switched off
+ * (main.cpp:247); mfp_bench runs it only when asked. */
+int mfp_gpu_hybrid_cpu_coo_spmv(int dtype, const mfp_host_ell *ell, const mfp_host_coo *coo,
+                                mfp_buffer A_ell, mfp_buffer ell_col_ids, mfp_buffer x,
+                                mfp_buffer y, mfp_buffer tmp, void *cpu_y, void *reusable_vector,
+                                const void *reference_y, mfp_measurement *out);
 
 /* ---- synthetic matrices (host generator, seeded, deterministic) ----------- */
 enum {

@@ -699,6 +699,27 @@ MFP_COMPAT_HYB_ENTRY(gpu_hybrid_spmv, mfp_gpu_hybrid_spmv)
 MFP_COMPAT_HYB_ENTRY(gpu_hybrid_atomic_spmv, mfp_gpu_hybrid_atomic_spmv)
 #undef MFP_COMPAT_HYB_ENTRY
 
+/* gpu_hybrid_cpu_coo_spmv (gpu/gpu_matrix_multiplier.h:141-152) */
+template <typename data_type>
+measurement_class gpu_hybrid_cpu_coo_spmv(const hybrid_matrix_class<data_type> &matrix,
+                                          resizable_gpu_memory<data_type> &A_ell,
+                                          resizable_gpu_memory<unsigned int> &ell_col_ids,
+                                          resizable_gpu_memory<data_type> &x, resizable_gpu_memory<data_type> &y,
+                                          resizable_gpu_memory<data_type> &tmp, data_type *cpu_y,
+                                          data_type *reusable_vector, const data_type *reference_y)
+{
+  if (!matrix.ell_matrix || !matrix.coo_matrix)
+    throw std::runtime_error("gpu_hybrid_cpu_coo_spmv: hybrid_matrix_class::allocate was not called");
+  const mfp_host_ell he = mfp_compat::host_ell(*matrix.ell_matrix);
+  const mfp_host_coo hc = mfp_compat::host_coo(*matrix.coo_matrix);
+  mfp_measurement m;
+  mfp_compat::check(mfp_gpu_hybrid_cpu_coo_spmv(mfp_compat::dtype_of<data_type>(), &he, &hc, A_ell.handle(),
+                                                ell_col_ids.handle(), x.handle(), y.handle(), tmp.handle(), cpu_y,
+                                                reusable_vector, reference_y, &m),
+                    "gpu_hybrid_cpu_coo_spmv");
+  return measurement_class(m);
+}
+
 /* ---- the reference's CPU measurements (cpu/cpu_matrix_multiplier.h) ------
  * Each sets x = 1 over cols, as the reference does. */
 namespace mfp_compat {

@@ -14,7 +14,7 @@ from .engine import (FORMATS, LABELS, Batch, autotune, DeviceArray, DeviceMatrix
 from .reference_api import (compare_results, coo_matrix_class, csr_matrix_class,  # noqa: E402
                             ell_matrix_class, fill_row_blocks, get_rows_statistics,
                             gpu_coo_spmv, gpu_csr_adaptive_spmv, gpu_csr_spmv,
-                            gpu_csr_vector_spmv, gpu_ell_spmv, gpu_hybrid_atomic_spmv,
+                            gpu_csr_vector_spmv, gpu_ell_spmv, gpu_hybrid_atomic_spmv, gpu_hybrid_cpu_coo_spmv,
                             gpu_hybrid_spmv, gpu_scoo_spmv, hybrid_matrix_class, matrix_class,
                             matrix_meta, matrix_rows_statistic, measurement_class,
                             resizable_gpu_memory, scoo_geometry, scoo_matrix_class)
@@ -26,7 +26,7 @@ __all__ = [
     "init", "residual", "synchronize", "workload", "compare_results", "coo_matrix_class", "csr_matrix_class",
     "ell_matrix_class", "fill_row_blocks", "get_rows_statistics", "gpu_coo_spmv",
     "gpu_csr_adaptive_spmv", "gpu_csr_spmv", "gpu_csr_vector_spmv", "gpu_ell_spmv",
-    "gpu_hybrid_atomic_spmv", "gpu_hybrid_spmv", "gpu_scoo_spmv", "hybrid_matrix_class",
+    "gpu_hybrid_atomic_spmv", "gpu_hybrid_cpu_coo_spmv", "gpu_hybrid_spmv", "gpu_scoo_spmv", "hybrid_matrix_class",
     "matrix_class", "matrix_meta", "matrix_rows_statistic", "measurement_class",
     "resizable_gpu_memory", "scoo_geometry", "scoo_matrix_class", "selector",
 ]

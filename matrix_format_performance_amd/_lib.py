@@ -190,6 +190,8 @@ SIGNATURES = {
                                   P(measurement)]),
     "mfp_gpu_hybrid_atomic_spmv": (i32, [i32, P(host_ell), P(host_coo), vp, vp, vp, vp, vp, vp, vp,
                                          vp, vp, P(measurement)]),
+    "mfp_gpu_hybrid_cpu_coo_spmv": (i32, [i32, P(host_ell), P(host_coo), vp, vp, vp, vp, vp, vp, vp, vp,
+                                          P(measurement)]),
     "mfp_generate": (i32, [P(gen_params), P(u32), P(u32), P(u64), vp, vp, vp, i32]),
     "mfp_generate_vector": (i32, [u64, u64, vp, i32]),
 }

@@ -22,6 +22,10 @@
 //   --no-cpu         omit the CPU rows (the reference answer is still computed)
 //   --no-rocsparse   omit the "GPU CSR (rocSPARSE)" row (left out anyway when
 //                    librocsparse.so cannot be opened)
+//   --hybrid-sweep   also run the block the reference keeps switched off
+//                    (main.cpp:247-279): hybrid ELL widths at 0%, 35% and 70%,
+//                    each timed once with the 2-kernel, atomic and host-COO
+//                    hybrids; rows of equal label keep the last width's number
 //
 // Synthetic names: stencil2d_256 (2D 5-point, 256 x 256), powerlaw_1m
 // (1,048,576 rows, power-law row lengths 1..50,000), stencil3d_128 (3D
@@ -91,7 +95,7 @@ std::string get_filename(const std::string &path)
 struct options {
   std::string out = ".";
   unsigned reps = 20;
-  bool meta_only = false, ell_skip = false, cpu = true, rocsparse = true;
+  bool meta_only = false, ell_skip = false, cpu = true, rocsparse = true, hybrid_sweep = false;
   std::vector<std::string> synthetic;
   std::string list;
 };
@@ -305,6 +309,23 @@ std::vector<measurement_class> perform_measurement(const matrix_market::matrix_c
                                        reference.get());
     });
   }
+  if (opt.hybrid_sweep) {
+    resizable_gpu_memory<T> A_coo, tmp;
+    resizable_gpu_memory<unsigned int> col_ids_coo;
+    hybrid_matrix_class<T> hybrid(csr);
+    auto once = [&](measurement_class m) {
+      out.push_back(m);
+      print(m);
+    };
+    for (double percent = 0.0; percent <= 1.0; percent += 0.35) {
+      hybrid.allocate(csr, percent);
+      std::printf("  hybrid sweep: %d%%, ELL width %u\n", (int)(percent * 100), hybrid.ell_matrix->elements_in_rows);
+      once(gpu_hybrid_spmv<T>(hybrid, A, A_coo, col_ids, col_ids_coo, row_ptr, x_gpu, y, x.get(), reference.get()));
+      once(gpu_hybrid_atomic_spmv<T>(hybrid, A, A_coo, col_ids, col_ids_coo, row_ptr, x_gpu, y, x.get(),
+                                     reference.get()));
+      once(gpu_hybrid_cpu_coo_spmv<T>(hybrid, A, col_ids, x_gpu, y, tmp, cpu_y.get(), x.get(), reference.get()));
+    }
+  }
   return out;
 }
 
@@ -394,6 +415,7 @@ int main(int argc, char **argv)
       else if (a == "--ell-skip") opt.ell_skip = true;
       else if (a == "--no-cpu") opt.cpu = false;
       else if (a == "--no-rocsparse") opt.rocsparse = false;
+      else if (a == "--hybrid-sweep") opt.hybrid_sweep = true;
       else if (a == "--synthetic") {
         std::stringstream ss(next());
         std::string s;
@@ -406,7 +428,7 @@ int main(int argc, char **argv)
     }
   }
   if (opt.synthetic.empty() && opt.list.empty()) {
-    std::cerr << "Usage: " << argv[0] << " [--out DIR] [--reps N] [--meta-only] [--ell-skip] [--no-cpu] [--no-rocsparse] "
+    std::cerr << "Usage: " << argv[0] << " [--out DIR] [--reps N] [--meta-only] [--ell-skip] [--no-cpu] [--no-rocsparse] [--hybrid-sweep] "
                  "(/path/to/mtx_list | --synthetic NAME[;NAME...])\n";
     return 1;
   }

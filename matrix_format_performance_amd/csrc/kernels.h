@@ -39,6 +39,7 @@ int launch_ell(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const v
                const void *x, void *y, hipStream_t s);
 int launch_coo_add(int dtype, uint64_t n, const uint32_t *cr, const uint32_t *cc, const void *cv,
                    const void *x, void *y, hipStream_t s);
+int launch_vec_add(int dtype, uint32_t n, const void *t, void *y, hipStream_t s);
 int launch_hyb_atomic(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const void *ev,
                       uint64_t ncoo, const uint32_t *cr, const uint32_t *cc, const void *cv,
                       const void *x, void *y, hipStream_t s);

@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstring>
 #include <iostream>
+#include <vector>
+#include <algorithm>
 
 using namespace mfp;
 
@@ -171,6 +173,37 @@ uint32_t default_lanes(uint32_t rows, uint64_t nnz)
   while (l < avg && l < 64) l <<= 1;
   return l;
 }
+
+// The hybrid's COO tail on the host, one entry at a time in stored order
+// (gpu_matrix_multiplier.cu:1163-1168); the product is rounded before the add.
+template <typename T>
+void host_coo_tail(const mfp_host_coo *C, const T *x, T *y)
+{
+#pragma clang fp contract(off)
+  const T *a = (const T *)C->data;
+  for (uint64_t e = 0; e < C->elements; e++) {
+    const T p = a[e] * x[C->cols_[e]];
+    y[C->rows_[e]] = y[C->rows_[e]] + p;
+  }
+}
+
+struct two_streams {
+  hipStream_t ell = nullptr, copy = nullptr;
+  hipEvent_t copied = nullptr;
+  ~two_streams()
+  {
+    if (copied) (void)hipEventDestroy(copied);
+    if (copy) (void)hipStreamDestroy(copy);
+    if (ell) (void)hipStreamDestroy(ell);
+  }
+  int create()
+  {
+    MFP_HIP(hipStreamCreateWithFlags(&ell, hipStreamNonBlocking));
+    MFP_HIP(hipStreamCreateWithFlags(&copy, hipStreamNonBlocking));
+    MFP_HIP(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
+    return MFP_OK;
+  }
+};
 
 } // namespace
 
@@ -389,6 +422,59 @@ int mfp_gpu_hybrid_atomic_spmv(int dtype, const mfp_host_ell *E, const mfp_host_
   MFP_CHECK(hybrid_common(dtype, E, C, Ae, Ac, ec, cc, cr, xd, yd, true, &sec));
   MFP_CHECK(finish(dtype, E->rows, mfp_buffer_ptr(yd), reusable_vector, reference_y));
   fill_measurement(out, "GPU Hybrid (atomic)", sec, 0.0, 0.0); // :1091-1095
+  return MFP_OK;
+}
+
+int mfp_gpu_hybrid_cpu_coo_spmv(int dtype, const mfp_host_ell *E, const mfp_host_coo *C,
+                                mfp_buffer Ae, mfp_buffer ec, mfp_buffer xd, mfp_buffer yd,
+                                mfp_buffer tmp, void *cpu_y, void *reusable_vector,
+                                const void *reference_y, mfp_measurement *out)
+{
+  MFP_CHECK(check_dtype(dtype));
+  MFP_ARG(E && C && out && tmp, "null pointer");
+  MFP_ARG(cpu_y || E->rows == 0, "cpu_y is null");
+  MFP_ARG(C->rows == E->rows && C->cols == E->cols, "the ELL and COO parts differ in shape");
+  for (uint64_t e = 0; e < C->elements; e++)
+    MFP_ARG(C->rows_[e] < E->rows && C->cols_[e] < E->cols, "COO index out of range");
+  const uint32_t rows = E->rows;
+  const size_t s = dtype_size(dtype);
+  MFP_CHECK(upload_ell(dtype, E, Ae, ec));
+  MFP_CHECK(xy_buffers(dtype, rows, E->cols, xd, yd, 0.0));
+  MFP_CHECK(mfp_buffer_resize(tmp, (size_t)rows * s));
+  // host x = 1 over cols, cpu_y = 0 over rows (gpu_matrix_multiplier.cu:1131-1133)
+  std::vector<double> x64;
+  std::vector<float> x32;
+  if (dtype == MFP_F32) {
+    x32.assign(E->cols, 1.0f);
+    std::fill_n((float *)cpu_y, rows, 0.0f);
+  } else {
+    x64.assign(E->cols, 1.0);
+    std::fill_n((double *)cpu_y, rows, 0.0);
+  }
+  two_streams st;
+  MFP_CHECK(st.create());
+  timed ev;
+  MFP_CHECK(ev.create());
+  MFP_HIP(hipDeviceSynchronize());
+  MFP_HIP(hipEventRecord(ev.a, st.ell));
+  // the ELL part runs on the GPU while the host sums the COO tail; the tail
+  // is copied on a second stream and added once both are done
+  MFP_CHECK(launch_ell(dtype, rows, E->elements_in_rows, (const uint32_t *)mfp_buffer_ptr(ec),
+                       mfp_buffer_ptr(Ae), mfp_buffer_ptr(xd), mfp_buffer_ptr(yd), st.ell));
+  if (dtype == MFP_F32)
+    host_coo_tail<float>(C, x32.data(), (float *)cpu_y);
+  else
+    host_coo_tail<double>(C, x64.data(), (double *)cpu_y);
+  MFP_HIP(hipMemcpyAsync(mfp_buffer_ptr(tmp), cpu_y, (size_t)rows * s, hipMemcpyHostToDevice, st.copy));
+  MFP_HIP(hipEventRecord(st.copied, st.copy));
+  MFP_HIP(hipStreamWaitEvent(st.ell, st.copied, 0));
+  MFP_CHECK(launch_vec_add(dtype, rows, mfp_buffer_ptr(tmp), mfp_buffer_ptr(yd), st.ell));
+  MFP_HIP(hipEventRecord(ev.b, st.ell));
+  MFP_HIP(hipEventSynchronize(ev.b));
+  float ms = 0;
+  MFP_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+  MFP_CHECK(finish(dtype, rows, mfp_buffer_ptr(yd), reusable_vector, reference_y));
+  fill_measurement(out, "GPU Hybrid (CPU COO TODO)", ms / 1000.0, 1.0, 1.0); // :1209-1213
   return MFP_OK;
 }
 

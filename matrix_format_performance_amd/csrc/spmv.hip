@@ -468,6 +468,15 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t rows, uint32_t w, const ui
   if (r < rows) ell_row<T, false>(r, rows, w, ec, ev, x, y);
 }
 
+// y += t, element-wise (the host-COO hybrid's merge of the tail sums,
+// gpu_matrix_multiplier.cu:1098-1108)
+template <typename T>
+__global__ __launch_bounds__(256) void k_vec_add(uint32_t n, const T *__restrict__ t, T *__restrict__ y)
+{
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+  if (i < n) y[i] = y[i] + t[i];
+}
+
 // ---------------------------------------------------------------------------
 // COO (gpu_matrix_multiplier.cu:622-638): a wave takes 4 x 64 consecutive
 // (row-sorted) entries, all loads in flight together.  Per 64-entry tile a
@@ -1105,6 +1114,15 @@ int launch_ell(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const v
                     (float *)y);
   return launch_k(k_ell<double>, dim3(grid), dim3(256), 0, s, rows, w, ec, (const double *)ev, (const double *)x,
                   (double *)y);
+}
+
+int launch_vec_add(int dtype, uint32_t n, const void *t, void *y, hipStream_t s)
+{
+  if (n == 0) return MFP_OK;
+  const uint32_t grid = ceil_div_u32(n, 256);
+  if (dtype == MFP_F32)
+    return launch_k(k_vec_add<float>, dim3(grid), dim3(256), 0, s, n, (const float *)t, (float *)y);
+  return launch_k(k_vec_add<double>, dim3(grid), dim3(256), 0, s, n, (const double *)t, (double *)y);
 }
 
 static uint32_t coo_grid(uint64_t n)

@@ -6,7 +6,8 @@ Same names, argument meaning and error behaviour as the reference's C++:
   measurement_class.h       measurement_class, compare_results
   resizable_gpu_memory.h    resizable_gpu_memory
   gpu_matrix_multiplier.h   gpu_csr_spmv, gpu_csr_vector_spmv, gpu_ell_spmv,
-                            gpu_coo_spmv, gpu_hybrid_spmv, gpu_hybrid_atomic_spmv
+                            gpu_coo_spmv, gpu_hybrid_spmv, gpu_hybrid_atomic_spmv,
+                            gpu_hybrid_cpu_coo_spmv
   csr_adaptive_spmv.h       gpu_csr_adaptive_spmv (+ fill_row_blocks)
   scoo_spmv.h               gpu_scoo_spmv
 The layouts are built by libmfp's device converters and the SpMVs run its HIP
@@ -447,3 +448,21 @@ def gpu_hybrid_atomic_spmv(matrix, A_ell, A_coo, ell_col_ids, coo_col_ids, coo_r
                            reusable_vector, reference_y):
     return _hyb_call("mfp_gpu_hybrid_atomic_spmv", matrix, A_ell, A_coo, ell_col_ids, coo_col_ids,
                      coo_row_ids, x, y, reusable_vector, reference_y)
+
+
+def gpu_hybrid_cpu_coo_spmv(matrix: hybrid_matrix_class, A_ell, ell_col_ids, x, y, tmp, cpu_y,
+                            reusable_vector, reference_y):
+    """gpu/gpu_matrix_multiplier.cu:1111-1217: the ELL part on the GPU while
+    the host sums the COO tail into ``cpu_y``; the tail goes through ``tmp``
+    and is added to y."""
+    he, hc = _host_ell(matrix.ell_matrix), _host_coo(matrix.coo_matrix)
+    dt = matrix.ell_matrix.data.dtype
+    cy = _check_vec(cpu_y, matrix.meta.rows_count, dt, "cpu_y")
+    if cy is None and matrix.meta.rows_count:
+        raise ValueError("cpu_y is required")
+    rv = _check_vec(reusable_vector, matrix.meta.rows_count, dt, "reusable_vector")
+    ry = _check_vec(reference_y, matrix.meta.rows_count, dt, "reference_y")
+    out = _meas()
+    L.call("mfp_gpu_hybrid_cpu_coo_spmv", E.dtype_code(dt), C.byref(he), C.byref(hc), A_ell.h,
+           ell_col_ids.h, x.h, y.h, tmp.h, cy, rv, ry, C.byref(out))
+    return measurement_class._from_c(out)

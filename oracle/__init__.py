@@ -58,6 +58,7 @@ def _load() -> C.CDLL:
         "orc_spmv_csr_seq_nofma": (None, [i32, u32, _u32p, _u32p, _vp, _vp, _vp]),
         "orc_spmv_csr_lanes": (None, [i32, u32, u32, _u32p, _u32p, _vp, _vp, _vp]),
         "orc_spmv_ell_seq": (None, [i32, u32, u32, _u32p, _vp, _vp, _vp]),
+        "orc_spmv_coo_seq": (None, [i32, u64, _u32p, _u32p, _vp, _vp, _vp]),
         "orc_spmv_csr_adaptive": (None, [i32, u32, _u32p, _u32p, _vp, _vp, _vp, u32, u32, u32, u32]),
         "orc_spmv_csr_f64": (None, [i32, u32, _u32p, _u32p, _vp, _vp, _f64p, _vp]),
         "orc_spmv_csr_multi_thread": (dbl, [i32, u32, u32, _u32p, _u32p, _vp, _vp, _vp]),
@@ -242,6 +243,23 @@ def spmv_ell(rows, width, ell_cols, ell_data, x):
     y = np.zeros(rows, ell_data.dtype)
     _lib.orc_spmv_ell_seq(_dt(ell_data), rows, width, ell_cols, _ptr(ell_data), _ptr(x), _ptr(y))
     return y
+
+
+def spmv_coo_seq(rows, coo_rows, coo_cols, coo_data, x, y=None):
+    """gpu/gpu_matrix_multiplier.cu:1163-1168 order (entry by entry, product
+    rounded before the add); accumulates into y (zeros when None)."""
+    coo_rows, coo_cols, coo_data = _c(coo_rows, np.uint32), _c(coo_cols, np.uint32), _c(coo_data)
+    x = _c(x, coo_data.dtype)
+    y = np.zeros(rows, coo_data.dtype) if y is None else np.array(y, coo_data.dtype, copy=True)
+    if coo_data.size:
+        _lib.orc_spmv_coo_seq(_dt(coo_data), coo_data.size, coo_rows, coo_cols, _ptr(coo_data), _ptr(x), _ptr(y))
+    return y
+
+
+def spmv_hybrid_host_coo(rows, width, ell_cols, ell_data, coo_rows, coo_cols, coo_data, x):
+    """gpu/gpu_matrix_multiplier.cu:1111-1217: y = ELL part (slot order) +
+    host COO tail (entry order), one rounding for the final add."""
+    return spmv_ell(rows, width, ell_cols, ell_data, x) + spmv_coo_seq(rows, coo_rows, coo_cols, coo_data, x)
 
 
 def spmv_adaptive(row_ptr, columns, data, x, nnz_per_wg, block_threads=256,

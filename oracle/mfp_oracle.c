@@ -415,6 +415,26 @@ void orc_spmv_csr_lanes(int dtype, uint32_t lanes, uint32_t rows,
     spmv_lanes_f64(lanes, rows, row_ptr, columns, data, x, y);
 }
 
+/* gpu/gpu_matrix_multiplier.cu:1163-1168: the host COO loop of the
+ * host-COO hybrid, y[row[e]] += data[e] * x[col[e]] in stored order, the
+ * product rounded before the add.  y is accumulated into, not cleared. */
+void orc_spmv_coo_seq(int dtype, uint64_t n, const uint32_t *coo_rows,
+                      const uint32_t *coo_cols, const void *coo_data,
+                      const void *x, void *y)
+{
+  for (uint64_t e = 0; e < n; e++) {
+    if (dtype == ORC_F32) {
+      float *yy = y;
+      yy[coo_rows[e]] = yy[coo_rows[e]] +
+                        mul_f32(((const float *)coo_data)[e], ((const float *)x)[coo_cols[e]]);
+    } else {
+      double *yy = y;
+      yy[coo_rows[e]] = yy[coo_rows[e]] +
+                        mul_f64(((const double *)coo_data)[e], ((const double *)x)[coo_cols[e]]);
+    }
+  }
+}
+
 void orc_spmv_ell_seq(int dtype, uint32_t rows, uint32_t width,
                       const uint32_t *ell_cols, const void *ell_data,
                       const void *x, void *y)

@@ -116,6 +116,9 @@ void orc_spmv_csr_lanes(int dtype, uint32_t lanes, uint32_t rows,
 
 /* ELL: thread per row, sequential over the padded width with fma
  * (gpu/gpu_matrix_multiplier.cu:450-471). */
+void orc_spmv_coo_seq(int dtype, uint64_t n, const uint32_t *coo_rows,
+                      const uint32_t *coo_cols, const void *coo_data,
+                      const void *x, void *y); /* gpu_matrix_multiplier.cu:1163-1168 */
 void orc_spmv_ell_seq(int dtype, uint32_t rows, uint32_t width,
                       const uint32_t *ell_cols, const void *ell_data,
                       const void *x, void *y);

@@ -119,6 +119,23 @@ def test_matrix_market_list_and_meta_mode(gpu, tmp_path):
     assert json.load(open(out2 / "float.json")) is None  # no measurements in this mode
 
 
+def test_hybrid_sweep_rows(gpu, tmp_path):
+    # --hybrid-sweep runs main.cpp:247-279 (switched off in the reference):
+    # three ELL widths x three hybrids, each timed once; the host-COO row has
+    # its own label and the reference's (1, 1) byte/flop record
+    r = _run(["--reps", "1", "--no-cpu", "--hybrid-sweep", "--synthetic", "powerlaw:rows=20000,lo=1,hi=3000,seed=4"],
+             tmp_path)
+    assert "ERROR" not in r.stderr, r.stderr[-2000:]
+    assert r.stdout.count("hybrid sweep:") == 6  # float and double
+    assert r.stdout.count("GPU Hybrid (CPU COO TODO)") == 6
+    js = _load(tmp_path)
+    for prec in ("float", "double"):
+        t = js[f"{prec}.json"]["powerlaw:rows=20000,lo=1,hi=3000,seed=4"]
+        bw = js[f"{prec}_effective_bandwidth.json"]["powerlaw:rows=20000,lo=1,hi=3000,seed=4"]
+        lab = "GPU Hybrid (CPU COO TODO)"
+        assert t[lab] > 0 and bw[lab] == pytest.approx(1.0 / (t[lab] * 1e9), rel=1e-12)
+
+
 def test_symmetric_matrix_is_refused(gpu, tmp_path):
     p = tmp_path / "sym.mtx"
     p.write_text("%%MatrixMarket matrix coordinate real symmetric\n2 2 2\n1 1 1.0\n2 1 2.0\n")

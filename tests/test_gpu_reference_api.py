@@ -76,6 +76,51 @@ def test_main_cpp_sequence(gpu, dtype):
     check(m, "GPU Hybrid (atomic)")
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_hybrid_host_coo_tail(gpu, dtype):
+    # main.cpp:247-279 (switched off there): the ELL widths of its percent
+    # loop, the host-COO hybrid against ELL slot order + the host's entry
+    # order, bit for bit
+    mfp = gpu
+    announce(f"reference API host-COO hybrid dtype={np.dtype(dtype)}")
+    csr = mfp.csr_matrix_class(_matrix(mfp, dtype), dtype)
+    rows = csr.meta.rows_count
+    ones = np.ones(csr.meta.cols_count, dtype)
+    reference_answer = oracle.spmv_seq(csr.row_ptr, csr.columns, csr.data, ones)
+    A, x_gpu, y, tmp = (mfp.resizable_gpu_memory(dtype) for _ in range(4))
+    col_ids = mfp.resizable_gpu_memory(np.uint32)
+    x = np.zeros(max(rows, csr.meta.cols_count), dtype)
+    cpu_y = np.full(rows, 7.0, dtype)  # zeroed by the call
+    hyb = mfp.hybrid_matrix_class(csr)
+    for percent in (0.0, 0.35, 0.7):
+        hyb.allocate(csr, percent)
+        e, c = hyb.ell_matrix, hyb.coo_matrix
+        assert e.elements_in_rows == oracle.hybrid_width(csr.row_ptr, percent)
+        m = mfp.gpu_hybrid_cpu_coo_spmv(hyb, A, col_ids, x_gpu, y, tmp, cpu_y, x, reference_answer)
+        assert m.get_format() == "GPU Hybrid (CPU COO TODO)" and m.get_elapsed() > 0
+        assert m.load_store_bytes == 1.0 and m.operations_count == 1.0  # gpu_matrix_multiplier.cu:1209-1213
+        tail = oracle.spmv_coo_seq(rows, c.rows, c.cols, c.data, ones)
+        assert np.array_equal(cpu_y, tail)
+        want = oracle.spmv_ell(rows, e.elements_in_rows, e.columns, e.data, ones) + tail
+        assert np.array_equal(x[:rows], want), percent
+        assert tmp.size_bytes() >= rows * np.dtype(dtype).itemsize
+
+
+def test_hybrid_host_coo_rejects_mismatched_parts(gpu):
+    mfp = gpu
+    csr = mfp.csr_matrix_class(_matrix(mfp, np.float32), np.float32)
+    hyb = mfp.hybrid_matrix_class(csr)
+    hyb.allocate(csr, 0.2)
+    hyb.coo_matrix.rows = hyb.coo_matrix.rows.copy()
+    if hyb.coo_matrix.rows.size:
+        hyb.coo_matrix.rows[-1] = csr.meta.rows_count  # one past the last row
+        A, x_gpu, y, tmp = (mfp.resizable_gpu_memory(np.float32) for _ in range(4))
+        col_ids = mfp.resizable_gpu_memory(np.uint32)
+        with pytest.raises(Exception, match="out of range"):
+            mfp.gpu_hybrid_cpu_coo_spmv(hyb, A, col_ids, x_gpu, y, tmp, np.zeros(csr.meta.rows_count, np.float32),
+                                        np.zeros(csr.meta.rows_count, np.float32), None)
+
+
 def test_vector_prefill_sentinel_and_averaging(gpu):
     mfp = gpu
     announce("reference API averaging")

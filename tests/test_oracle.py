@@ -339,6 +339,42 @@ def test_ell_order_is_sequential():
     assert np.array_equal(ye, ys)
 
 
+def test_host_coo_order_is_unfused_row_order():
+    # the full COO (element_start 0) visits each row's entries in CSR order,
+    # so the entry-order loop with rounded products is the unfused CSR loop
+    r, c, v = _random_csr(300, 250, 20, seed=21)
+    x = np.random.default_rng(6).uniform(-1, 1, 250)
+    for dt in (np.float32, np.float64):
+        rp, ci, va = oracle.csr_from_triplets(300, 250, r, c, v, dt)
+        cd, cc, cr = oracle.coo_from_csr(rp, ci, va)
+        yc = oracle.spmv_coo_seq(300, cr, cc, cd, x.astype(dt))
+        assert np.array_equal(yc, oracle.spmv_seq_nofma(rp, ci, va, x.astype(dt)))
+        assert not np.array_equal(yc, oracle.spmv_seq(rp, ci, va, x.astype(dt)))  # fused differs somewhere
+    # accumulates into a given y instead of clearing it
+    y0 = np.arange(3, dtype=np.float64)
+    got = oracle.spmv_coo_seq(3, [2, 0], [0, 1], [2.0, 5.0], np.array([3.0, 7.0]), y=y0)
+    assert got.tolist() == [35.0, 1.0, 8.0] and y0.tolist() == [0.0, 1.0, 2.0]
+
+
+@pytest.mark.parametrize("w", [0, 3, 9, 40])
+def test_hybrid_host_coo_integer_closed_form(w):
+    # integer values and x: every partial sum is exact, so ELL(w) + tail(w)
+    # equals the integer row sums whatever the split
+    rng = np.random.default_rng(22)
+    r, c, _ = _random_csr(200, 200, 30, seed=23)
+    v = rng.integers(-8, 9, r.size).astype(np.float64)
+    x = rng.integers(-4, 5, 200).astype(np.float32)
+    rp, ci, va = oracle.csr_from_triplets(200, 200, r, c, v, np.float32)
+    _, ed, ec = oracle.ell_from_csr(rp, ci, va, width=w)
+    cd, cc, cr = oracle.coo_from_csr(rp, ci, va, element_start=w)
+    y = oracle.spmv_hybrid_host_coo(200, w, ec, ed, cr, cc, cd, x)
+    want = np.zeros(200, np.int64)
+    np.add.at(want, np.repeat(np.arange(200), np.diff(rp)), va.astype(np.int64) * x.astype(np.int64)[ci])
+    assert np.array_equal(y.astype(np.int64), want)
+    if w == 40:  # wider than every row: the tail is empty, the ELL sum alone
+        assert cd.size == 0 and np.array_equal(y, oracle.spmv_ell(200, w, ec, ed, x))
+
+
 def test_multi_thread_matches_single_thread():
     r, c, v = _random_csr(1000, 1000, 30, seed=17)
     rp, ci, va = oracle.csr_from_triplets(1000, 1000, r, c, v, np.float32)
