@@ -45,10 +45,28 @@ template <> struct vec16<double> {
 // ---------------------------------------------------------------------------
 // CSR-scalar: one lane per row, as gpu_matrix_multiplier.cu:22-43, but the
 // wave stages its rows' contiguous nnz range through LDS with coalesced loads
-// (all of a chunk's loads issued before use) so each lane then walks only its
-// own row.  Order: sequential fma.
+// so each lane then walks only its own row.  Order: sequential fma.
+// The staging of a chunk is split in two so that no load waits behind
+// another: the values and column indices of chunk k + 1 are issued before
+// chunk k is summed, and their x gathers right after; every load sits at a
+// clamped in-range offset, so none is predicated.  A lane reads its row
+// from LDS eight elements at a time, so a long row pays the LDS latency once
+// per eight fma.
 // ---------------------------------------------------------------------------
 constexpr uint32_t SCALAR_PER_LANE = 8, SCALAR_CHUNK = 64 * SCALAR_PER_LANE;
+
+template <typename T>
+__device__ __forceinline__ void scalar_stage_loads(const uint32_t *__restrict__ ci, const T *__restrict__ a,
+                                                   uint32_t c0, uint32_t n, uint32_t lane, T (&av)[SCALAR_PER_LANE],
+                                                   uint32_t (&cv)[SCALAR_PER_LANE])
+{
+#pragma unroll
+  for (uint32_t k = 0; k < SCALAR_PER_LANE; k++) {
+    const uint32_t j = min(lane + 64u * k, n - 1u); // n >= 1: a duplicate of the last element past the end
+    av[k] = ldg_stream(a + c0 + j);
+    cv[k] = ldg_stream(ci + c0 + j);
+  }
+}
 
 template <typename T>
 __global__ __launch_bounds__(64) void k_csr_scalar(uint32_t rows, const uint32_t *__restrict__ rp,
@@ -56,8 +74,8 @@ __global__ __launch_bounds__(64) void k_csr_scalar(uint32_t rows, const uint32_t
                                                    const T *__restrict__ a, const T *__restrict__ x,
                                                    T *__restrict__ y)
 {
-  __shared__ T va[SCALAR_CHUNK];
-  __shared__ T vx[SCALAR_CHUNK];
+  __shared__ T va[2][SCALAR_CHUNK];
+  __shared__ T vx[2][SCALAR_CHUNK];
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t r0 = blk * 64u;
   const uint32_t lane = threadIdx.x;
@@ -67,28 +85,46 @@ __global__ __launch_bounds__(64) void k_csr_scalar(uint32_t rows, const uint32_t
   const uint32_t re = rp[min(r + 1u, rows)];
   const uint32_t b = rp[r0], e = rp[rend];
   T dot = 0;
-  for (uint32_t c0 = b; c0 < e; c0 += SCALAR_CHUNK) {
-    const uint32_t n = min(SCALAR_CHUNK, e - c0);
-    T av[SCALAR_PER_LANE];
+  if (b < e) {
+    T av[SCALAR_PER_LANE], xv[SCALAR_PER_LANE];
     uint32_t cv[SCALAR_PER_LANE];
+    scalar_stage_loads(ci, a, b, min(SCALAR_CHUNK, e - b), lane, av, cv);
 #pragma unroll
-    for (uint32_t k = 0; k < SCALAR_PER_LANE; k++) {
-      const uint32_t j = lane + 64u * k;
-      av[k] = j < n ? ldg_stream(a + c0 + j) : (T)0;
-      cv[k] = j < n ? ldg_stream(ci + c0 + j) : 0u;
-    }
+    for (uint32_t k = 0; k < SCALAR_PER_LANE; k++) xv[k] = x[cv[k]];
+    uint32_t buf = 0;
+    for (uint32_t c0 = b; c0 < e; c0 += SCALAR_CHUNK, buf ^= 1u) {
+      const uint32_t n = min(SCALAR_CHUNK, e - c0);
+      // slots past n hold duplicates and are never read
 #pragma unroll
-    for (uint32_t k = 0; k < SCALAR_PER_LANE; k++) {
-      const uint32_t j = lane + 64u * k;
-      if (j < n) {
-        va[j] = av[k];
-        vx[j] = x[cv[k]];
+      for (uint32_t k = 0; k < SCALAR_PER_LANE; k++) {
+        va[buf][lane + 64u * k] = av[k];
+        vx[buf][lane + 64u * k] = xv[k];
       }
+      __syncthreads();
+      const uint32_t c1 = c0 + SCALAR_CHUNK;
+      const bool next = c1 < e; // uniform over the wave
+      if (next) scalar_stage_loads(ci, a, c1, min(SCALAR_CHUNK, e - c1), lane, av, cv);
+      const uint32_t lo = max(rs, c0), hi = min(re, c0 + n);
+      const T *pa = va[buf] - c0, *px = vx[buf] - c0;
+      uint32_t i = lo;
+      for (; i + 8u <= hi; i += 8u) {
+        T qa[8], qx[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) {
+          qa[q] = pa[i + q];
+          qx[q] = px[i + q];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) dot = fma_t(qa[q], qx[q], dot);
+      }
+      for (; i < hi; i++) dot = fma_t(pa[i], px[i], dot);
+      if (next) {
+#pragma unroll
+        for (uint32_t k = 0; k < SCALAR_PER_LANE; k++) xv[k] = x[cv[k]];
+      }
+      // the buffer written next is the one summed two chunks ago: the
+      // barrier above already ordered those reads
     }
-    __syncthreads();
-    const uint32_t lo = max(rs, c0), hi = min(re, c0 + n);
-    for (uint32_t i = lo; i < hi; i++) dot = fma_t(va[i - c0], vx[i - c0], dot);
-    __syncthreads();
   }
   if (r < rows) y[r] = dot;
 }
@@ -98,12 +134,18 @@ __global__ __launch_bounds__(64) void k_csr_scalar(uint32_t rows, const uint32_t
 // gpu_matrix_multiplier.cu:331-362).  Lane l accumulates elements l, l+G, ...
 // with fma (four elements' loads issued together), then group_reduce<G>.
 // ---------------------------------------------------------------------------
+// Rows with at least eight elements per lane run a software pipeline: the
+// values and column indices of the next eight are loaded (at clamped in-row
+// offsets, so no load is predicated) before the x entries of the current
+// eight are gathered and summed, so a long row waits for one memory latency
+// per eight elements instead of two per four.
 template <typename T, int G>
 __global__ __launch_bounds__(256) void k_csr_vector(uint32_t rows, const uint32_t *__restrict__ rp,
                                                     const uint32_t *__restrict__ ci,
                                                     const T *__restrict__ a, const T *__restrict__ x,
                                                     T *__restrict__ y)
 {
+  constexpr uint32_t U = 8;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t row = (blk * 256u + threadIdx.x) / G;
   const uint32_t lane = threadIdx.x % G;
@@ -111,6 +153,39 @@ __global__ __launch_bounds__(256) void k_csr_vector(uint32_t rows, const uint32_
   if (row < rows) {
     const uint32_t b = rp[row], e = rp[row + 1];
     uint32_t i = b + lane;
+    if (i + (U - 1) * G < e) {
+      T an[U];
+      uint32_t cn[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        an[u] = ldg_stream(a + i + u * G);
+        cn[u] = ldg_stream(ci + i + u * G);
+      }
+      for (;;) {
+        T ac[U];
+        uint32_t cc[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+          ac[u] = an[u];
+          cc[u] = cn[u];
+        }
+        const uint32_t inext = i + U * G;
+        const bool more = inext + (U - 1) * G < e;
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+          const uint32_t j = min(inext + u * G, e - 1u); // e > i: in range
+          an[u] = ldg_stream(a + j);
+          cn[u] = ldg_stream(ci + j);
+        }
+        T xv[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) xv[u] = x[cc[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) dot = fma_t(ac[u], xv[u], dot);
+        i = inext;
+        if (!more) break;
+      }
+    }
     for (; i + 3u * G < e; i += 4u * G) {
       const T a0 = ldg_stream(a + i), a1 = ldg_stream(a + i + G);
       const T a2 = ldg_stream(a + i + 2 * G), a3 = ldg_stream(a + i + 3 * G);
