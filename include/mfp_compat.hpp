@@ -757,24 +757,21 @@ measurement_class cpu_csr_spmv_single_thread_naive(const csr_matrix_class<data_t
                            2.0 * matrix.meta.non_zero_count);
 }
 
-/* cpu_matrix_multiplier.cpp:170-227, generalised to the lane count the
- * library's CSR-vector entry point uses for this matrix (the smallest power of
- * two >= ceil(nnz / rows), in [2, 64]; the reference's warp has 32): lane l
- * accumulates elements l, l + lanes, ... with fma, then lanes are added in a
- * halving tree (lane i += lane i + offset). */
+/* cpu_matrix_multiplier.cpp:170-227: 32 lanes, as the GPU CSR-vector entry
+ * point and the reference's warp; lane l accumulates elements l, l + 32, ...
+ * with fma, then lanes are added in a halving tree (lane i += lane i + offset,
+ * offsets 16, 8, 4, 2, 1). */
 template <typename data_type>
 measurement_class cpu_csr_spmv_single_thread_naive_with_reduce_order(const csr_matrix_class<data_type> &matrix,
                                                                      data_type *x, data_type *y)
 {
   std::fill_n(x, matrix.meta.cols_count, data_type(1.0));
   const unsigned rows = matrix.meta.rows_count;
-  const uint64_t avg = rows ? ((uint64_t)matrix.meta.non_zero_count + rows - 1) / rows : 1;
-  unsigned lanes = 2;
-  while (lanes < avg && lanes < 64) lanes <<= 1;
+  constexpr unsigned lanes = 32;
   const auto *rp = matrix.row_ptr.get();
   const auto *ci = matrix.columns.get();
   const auto *va = matrix.data.get();
-  data_type dots[64];
+  data_type dots[lanes];
   const auto begin = std::chrono::steady_clock::now();
   for (unsigned row = 0; row < rows; row++) {
     for (unsigned l = 0; l < lanes; l++) {

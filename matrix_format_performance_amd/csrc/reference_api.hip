@@ -166,13 +166,9 @@ int xy_buffers(int dtype, uint32_t rows, uint32_t cols, mfp_buffer xd, mfp_buffe
   return MFP_OK;
 }
 
-uint32_t default_lanes(uint32_t rows, uint64_t nnz)
-{
-  const uint64_t avg = rows ? (nnz + rows - 1) / rows : 1;
-  uint32_t l = 2;
-  while (l < avg && l < 64) l <<= 1;
-  return l;
-}
+// One 32-lane group per row, as the reference's warp (gpu_matrix_multiplier.cu:331-362,
+// launched with warp_size = 32 at :412-417)
+constexpr uint32_t REFERENCE_VECTOR_LANES = 32;
 
 // The hybrid's COO tail on the host, one entry at a time in stored order
 // (gpu_matrix_multiplier.cu:1163-1168); the product is rounded before the add.
@@ -243,10 +239,9 @@ int mfp_gpu_csr_vector_spmv(int dtype, const mfp_host_csr *A, mfp_buffer Ad, mfp
   MFP_ARG(out, "out is null");
   // y pre-filled with 24 so an unwritten row shows (gpu_matrix_multiplier.cu:403)
   MFP_CHECK(upload_csr(dtype, A, Ad, cd, rd, xd, yd, 24.0));
-  const uint32_t lanes = default_lanes(A->rows, A->nnz);
   double sec = 0;
   MFP_CHECK(run_timed([&] {
-    return launch_csr_vector(dtype, lanes, A->rows, (const uint32_t *)mfp_buffer_ptr(rd),
+    return launch_csr_vector(dtype, REFERENCE_VECTOR_LANES, A->rows, (const uint32_t *)mfp_buffer_ptr(rd),
                              (const uint32_t *)mfp_buffer_ptr(cd), mfp_buffer_ptr(Ad),
                              mfp_buffer_ptr(xd), mfp_buffer_ptr(yd), nullptr);
   }, &sec));

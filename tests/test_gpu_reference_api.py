@@ -49,10 +49,8 @@ def test_main_cpp_sequence(gpu, dtype):
     # the CSR byte model (gpu_matrix_multiplier.cu:120-124)
     assert m.load_store_bytes == oracle.reference_bytes(oracle.REF_BYTES_CSR, dtype, rows, csr.meta.non_zero_count)
     m = mfp.gpu_csr_vector_spmv(csr, A, col_ids, row_ptr, x_gpu, y, x, reference_answer)
-    lanes = 2
-    while lanes < -(-csr.meta.non_zero_count // rows) and lanes < 64:
-        lanes *= 2
-    check(m, "GPU CSR (vector)", exact=oracle.spmv_lanes(csr.row_ptr, csr.columns, csr.data, ones, lanes))
+    # the reference's warp: 32 lanes per row (gpu_matrix_multiplier.cu:331-362)
+    check(m, "GPU CSR (vector)", exact=oracle.spmv_lanes(csr.row_ptr, csr.columns, csr.data, ones, 32))
     m = mfp.gpu_csr_adaptive_spmv(csr, A, col_ids, row_ptr, x_gpu, y, x, reference_answer)
     W = mfp.engine.DEFAULT_NNZ_PER_WG[np.dtype(dtype)]
     check(m, "GPU CSR-Adaptive", exact=oracle.spmv_adaptive(csr.row_ptr, csr.columns, csr.data, ones, W, 256,
