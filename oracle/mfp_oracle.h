@@ -114,11 +114,15 @@ void orc_spmv_csr_lanes(int dtype, uint32_t lanes, uint32_t rows,
                         const uint32_t *row_ptr, const uint32_t *columns,
                         const void *data, const void *x, void *y);
 
-/* ELL: thread per row, sequential over the padded width with fma
- * (gpu/gpu_matrix_multiplier.cu:450-471). */
+/* Host COO loop of the host-COO hybrid (gpu/gpu_matrix_multiplier.cu:
+ * 1163-1168): y[row[e]] += data[e] * x[col[e]] in stored order, the product
+ * rounded before the add; y is accumulated into. */
 void orc_spmv_coo_seq(int dtype, uint64_t n, const uint32_t *coo_rows,
                       const uint32_t *coo_cols, const void *coo_data,
-                      const void *x, void *y); /* gpu_matrix_multiplier.cu:1163-1168 */
+                      const void *x, void *y);
+
+/* ELL: thread per row, sequential over the padded width with fma
+ * (gpu/gpu_matrix_multiplier.cu:450-471). */
 void orc_spmv_ell_seq(int dtype, uint32_t rows, uint32_t width,
                       const uint32_t *ell_cols, const void *ell_data,
                       const void *x, void *y);
