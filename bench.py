@@ -226,7 +226,7 @@ def kernel_bytes(fmt, info, nbytes, rows, s_y, nk):
 
 def rocsparse_row(mfp, A, xd, yd, rp, ci, va, x, nbytes, reps, flush, timer):
     """The "GPU CSR (rocSPARSE)" comparator (the reference's cuSPARSE slot,
-    gpu_matrix_multiplier.cu:135-257): rocsparse_spmv's CSR-adaptive
+    gpu_matrix_multiplier.cu:175-257): rocsparse_spmv's CSR-adaptive
     algorithm on the same device CSR, timed like the candidates (cache
     flush, event pair, median).  Benchmark-only: librocsparse.so is opened at
     run time by app/rocsparse_slot.cpp; None without it."""

@@ -1,6 +1,6 @@
 /*
  * Benchmark-only comparator: CSR SpMV by rocSPARSE, the counterpart of the
- * reference's "GPU CSR (cuSparse)" row (gpu/gpu_matrix_multiplier.cu:135-257,
+ * reference's "GPU CSR (cuSparse)" row (gpu/gpu_matrix_multiplier.cu:175-257,
  * main.cpp:194-197).  librocsparse.so is opened at run time (dlopen), so it
  * is never a link dependency of libmfp or of the driver, and no SpMV of the
  * library goes through it.  When the library is absent the slot reports

@@ -1,5 +1,5 @@
 """The benchmark-only rocSPARSE comparator slot (the counterpart of the
-reference's "GPU CSR (cuSparse)" row, gpu_matrix_multiplier.cu:135-257): its
+reference's "GPU CSR (cuSparse)" row, gpu_matrix_multiplier.cu:175-257): its
 y against the oracle within the per-row tolerance.  Skipped when
 librocsparse.so cannot be opened; the library's own SpMV never uses it."""
 import ctypes as C
