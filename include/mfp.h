@@ -262,6 +262,10 @@ int mfp_batch_matrix(mfp_batch b, mfp_matrix *out);
 int mfp_batch_spmm(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_stream s);
 /* capture that launch sequence once in a HIP graph; replay with mfp_graph_launch */
 int mfp_batch_graph_create(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_graph *out);
+/* capture one mfp_plan_spmv(p, x, y) in a HIP graph (the plan, x and y must
+ * outlive it); replay with mfp_graph_launch.  No reference counterpart: the
+ * reference launches each kernel from the host every call. */
+int mfp_plan_graph_create(mfp_plan p, const void *x, void *y, mfp_graph *out);
 int mfp_graph_launch(mfp_graph g, mfp_stream s);
 int mfp_graph_destroy(mfp_graph g);
 int mfp_batch_destroy(mfp_batch b);

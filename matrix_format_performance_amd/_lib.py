@@ -160,6 +160,7 @@ SIGNATURES = {
     "mfp_batch_matrix": (i32, [vp, P(vp)]),
     "mfp_batch_spmm": (i32, [vp, u32, vp, vp, vp]),
     "mfp_batch_graph_create": (i32, [vp, u32, vp, vp, P(vp)]),
+    "mfp_plan_graph_create": (i32, [vp, vp, vp, P(vp)]),
     "mfp_graph_launch": (i32, [vp, vp]),
     "mfp_graph_destroy": (i32, [vp]),
     "mfp_batch_destroy": (i32, [vp]),

@@ -491,6 +491,31 @@ int mfp_batch_graph_create(mfp_batch b, uint32_t k, const void *X, void *Y, mfp_
   return MFP_OK;
 }
 
+int mfp_plan_graph_create(mfp_plan p, const void *x, void *y, mfp_graph *out)
+{
+  MFP_ARG(p && x && y && out, "null pointer");
+  MFP_CHECK(ensure_init());
+  hipStream_t cs = nullptr;
+  MFP_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  auto *g = new mfp_graph_s;
+  int st = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess ? MFP_OK : MFP_ERR_HIP;
+  if (st == MFP_OK) {
+    st = mfp_plan_spmv(p, x, y, cs);
+    const hipError_t e = hipStreamEndCapture(cs, &g->graph);
+    if (st == MFP_OK && e != hipSuccess) st = MFP_ERR_HIP;
+  }
+  if (st == MFP_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    st = MFP_ERR_HIP;
+  (void)hipStreamDestroy(cs);
+  if (st != MFP_OK) {
+    if (st == MFP_ERR_HIP) set_error("graph capture of the plan's SpMV failed");
+    mfp_graph_destroy(g);
+    return st;
+  }
+  *out = g;
+  return MFP_OK;
+}
+
 int mfp_graph_launch(mfp_graph g, mfp_stream sv)
 {
   MFP_ARG(g && g->exec, "null graph");

@@ -248,6 +248,13 @@ class Plan:
     def spmv(self, x: DeviceArray, y: DeviceArray, stream=None) -> None:
         L.call("mfp_plan_spmv", self.handle, x.ptr, y.ptr, stream)
 
+    def graph(self, x: DeviceArray, y: DeviceArray) -> "Graph":
+        """mfp_plan_graph_create: one SpMV of this plan captured in a HIP graph
+        (replay with Graph.launch; the plan, x and y must outlive it)."""
+        h = C.c_void_p()
+        L.call("mfp_plan_graph_create", self.handle, x.ptr, y.ptr, C.byref(h))
+        return Graph(h)
+
     def spmv_noalloc(self, x: DeviceArray, y: DeviceArray, stream=None) -> None:
         """mfp_plan_spmv_noalloc: partial results in the library's static
         scratch (no allocation; overlapping calls on one device race)."""
@@ -324,7 +331,7 @@ def autotune(matrix: DeviceMatrix, x: DeviceArray, y: DeviceArray, formats=None,
 
 
 class Graph:
-    """An instantiated HIP graph of a batched launch."""
+    """An instantiated HIP graph (a batched launch or one plan's SpMV)."""
 
     def __init__(self, handle):
         self.handle = handle

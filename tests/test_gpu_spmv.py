@@ -360,6 +360,39 @@ def test_plan_kernel_times(gpu, fmt, tuning, name, kernels, ordered, dtype):
     A.destroy()
 
 
+@pytest.mark.parametrize("fmt,tuning,name,kernels,ordered", KERNEL_TIME_CASES)
+def test_plan_graph_replays(gpu, fmt, tuning, name, kernels, ordered):
+    # one plan's SpMV captured in a HIP graph (zero fills are kernels, never
+    # hipMemsetAsync): three replays, each after eager work on the same
+    # stream that overwrites y, equal to the eager SpMV
+    rp, ci, va, cols = case(name, "float32")
+    announce(f"plan graph fmt={fmt} tuning={tuning} case={name}")
+    x = gpu.generate_vector(9, cols, np.float32)
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, fmt, **tuning)
+    xd = gpu.DeviceArray.from_host(x)
+    y = gpu.DeviceArray(rp.size - 1, np.float32)
+    plan.spmv(xd, y)
+    gpu.synchronize()
+    y_eager = y.to_host()
+    g = plan.graph(xd, y)
+    _, absdot = oracle.spmv_f64(rp, ci, va, x)
+    for _ in range(3):
+        y.fill(np.nan)
+        plan.spmv(xd, y)  # eager work between replays
+        y.fill(np.nan)
+        g.launch()
+        gpu.synchronize()
+        y_graph = y.to_host()
+        if ordered:
+            assert np.array_equal(y_graph, y_eager)
+        else:
+            assert oracle.tolerance_check(y_graph, y_eager, absdot, 1e-5) <= 1e-5
+    g.destroy()
+    plan.destroy()
+    A.destroy()
+
+
 def test_plan_kernel_times_bf16_binned(gpu):
     # the binned 16-bit CSR launches one kernel per non-empty length bin, the
     # long-row chunks and their finish: an event after each, at most eight
