@@ -353,7 +353,7 @@ def precision_study(mfp, timer, flush, reps, cache):
         rp, ci, va = cache[wl] if wl in cache else mfp.workload(wl, np.float32)
         rows, nnz = rp.size - 1, ci.size
         A = mfp.DeviceMatrix.from_csr(rp, ci, va, rows)
-        x = mfp.generate_vector(1, rows, np.float32)
+        x = mfp.generate_vector(0, rows, np.float32)
         xd = mfp.DeviceArray.from_host(x)
         yd = mfp.DeviceArray(rows, np.float32)
         y64, ad = A.oracle_f64(xd)
@@ -423,7 +423,7 @@ def f64_leg(mfp, timer, flush, reps):
     rp, ci, va = mfp.workload("powerlaw_1m", np.float64)
     rows, nnz = rp.size - 1, ci.size
     A = mfp.DeviceMatrix.from_csr(rp, ci, va, rows)
-    xd = mfp.DeviceArray.from_host(mfp.generate_vector(1, rows, np.float64))
+    xd = mfp.DeviceArray.from_host(mfp.generate_vector(0, rows, np.float64))
     yd = mfp.DeviceArray(rows, np.float64)
     cands = mfp.autotune(A, xd, yd, reps=reps, flush=flush)
     ok = [c for c in cands if c["status"] == 0 and np.isfinite(c["ms"])]
@@ -524,7 +524,7 @@ def main():
         rp, ci, va = mfp.workload(args.workload, dtype)
     rows, nnz = rp.size - 1, ci.size
     cols = rows
-    x = mfp.generate_vector(1, cols, dtype)
+    x = mfp.generate_vector(0, cols, dtype)  # x ~ U[-1, 1) from seed 0
     nbytes = b_alg(rows, cols, nnz, 2 if args.values else s, s)
     log(f"[bench] workload {args.workload}: rows={rows} nnz={nnz} B_alg={nbytes / 1e6:.1f} MB "
         f"(generated in {time.time() - t0:.1f}s)")
