@@ -80,7 +80,7 @@ def run_batch(mfp, args, timer):
     tot_rows = sum(m[0].size - 1 for m in members)
     tot_nnz = sum(m[1].size for m in members)
     mean_lanes = int(min(64, max(2, 1 << max(0, int(np.ceil(np.log2(max(1.0, tot_nnz / tot_rows))))))))
-    modes = {"mean": mean_lanes, "binned": 0, "colsort": 1}
+    modes = {"mean": mean_lanes, "binned": 0, "colsort": 1, "stream": mfp._lib.MFP_BATCH_STREAM}
     if mean_lanes >= 4:
         modes["half"] = mean_lanes // 2
     if args.batch_lanes > 1:
@@ -89,6 +89,8 @@ def run_batch(mfp, args, timer):
         modes = {"binned": 0}
     elif args.batch_lanes == 1:
         modes = {"colsort": 1}
+    elif args.batch_lanes == mfp._lib.MFP_BATCH_STREAM:
+        modes = {"stream": mfp._lib.MFP_BATCH_STREAM}
     # k = 1: CSR-Adaptive (the first batch) or the column-sorted SCOO (the last)
     cs = {"flags": mfp._lib.MFP_TUNE_SCOO_COLSORT, "scoo_slice_size": 16384, "scoo_pieces": 2,
           "scoo_threads": 1024}
@@ -137,7 +139,8 @@ def run_batch(mfp, args, timer):
                            "colsort": "column-sorted SCOO" + (", 16384-row slices" if k == 1 else " SpMM"),
                            "mean": f"CSR-vector SpMM, {mean_lanes} lanes",
                            "half": f"CSR-vector SpMM, {mean_lanes // 2} lanes",
-                           "binned": "CSR-vector SpMM, lanes per row (2..64)"
+                           "binned": "CSR-vector SpMM, lanes per row (2..64)",
+                           "stream": "CSR-stream SpMM, row blocks of 512 entries"
                            }.get(mode, f"CSR-vector SpMM, {args.batch_lanes} lanes"),
                   "ms_by_plan": {n: round(v, 5) for n, v in timed.items()}}
         if k == 1:
@@ -491,9 +494,9 @@ def main():
     ap.add_argument("--traffic", default=None, help="json with FETCH/WRITE-derived bytes per launch")
     ap.add_argument("--no-precision", action="store_true", help="skip the BF16 / FP16 study")
     ap.add_argument("--batch-lanes", type=int, default=-1,
-                    help="batch512, k > 1: -1 = time the mean-based lane count, per-row lanes and the "
-                         "column-sorted SpMM and keep the fastest; 0 = per-row lanes; 1 = column-sorted SpMM; "
-                         "N = N lanes for every row")
+                    help="batch512, k > 1: -1 = time the mean-based lane count (and half of it), per-row "
+                         "lanes, the column-sorted SpMM and the CSR-stream SpMM and keep the fastest; 0 = "
+                         "per-row lanes; 1 = column-sorted SpMM; 3 = CSR-stream SpMM; N = N lanes for every row")
     ap.add_argument("--values", default=None, choices=["bf16", "f16"],
                     help="store the matrix values in 16 bits (FP32 vectors and accumulation)")
     ap.add_argument("--bandwidth", type=float, default=1.0,

@@ -226,11 +226,14 @@ int mfp_plan_spmv_noalloc(mfp_plan p, const void *x, void *y, mfp_stream s);
 /* Y = A X for k right-hand sides (k in {1,2,4,8,16}); X is cols x k and Y
  * rows x k, row-major; X is 16-byte aligned when k values make a multiple
  * of 16 bytes.  k > 1 runs on CSR-vector plans (f32 / f64; each column is
- * summed in the CSR-vector order) and on column-sorted SCOO plans (f32 /
- * f64, while scoo_slice_size * k doubles fit in the 160 KB LDS: k <= 4 for
- * 4096-row slices, 16 for 1024; exact products added in double per slice
- * row and column, partial slots added in slot order).  No reference
- * counterpart (the reference multiplies one vector). */
+ * summed in the CSR-vector order), on CSR-Adaptive plans (f32 / f64, the
+ * CSR-stream SpMM of MFP_BATCH_STREAM over row blocks of 512 entries; k
+ * values must fill whole 16-byte pieces, X and Y 16-byte aligned) and on
+ * column-sorted SCOO plans (f32 / f64, while scoo_slice_size * k doubles fit
+ * in the 160 KB LDS: k <= 4 for 4096-row slices, 16 for 1024; exact
+ * products added in double per slice row and column, partial slots added in
+ * slot order).  No reference counterpart (the reference multiplies one
+ * vector). */
 int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream s);
 
 /* ---- batches: many matrices in one launch ----------------------------------
@@ -246,7 +249,13 @@ typedef struct mfp_graph_s *mfp_graph;
  * column-sorted SCOO SpMM over the assembled matrix (one plan per k, built by
  * the first mfp_batch_spmm or mfp_batch_graph_create with that k: the
  * largest power-of-two slice whose k doubles per row fit in 160 KB of LDS,
- * one work item per slice and column band, 1,024 threads). */
+ * one work item per slice and column band, 1,024 threads); MFP_BATCH_STREAM
+ * for the CSR-stream SpMM: row blocks of whole rows with at most 512 entries
+ * (fill_row_blocks, a longer row alone), staged in LDS, each group of lanes
+ * that covers one X row summing 512 / groups consecutive entries, rows that
+ * cross groups finished by the group of their last entry (k values must fill
+ * whole 16-byte pieces: k >= 4 for f32, >= 2 for f64). */
+#define MFP_BATCH_STREAM 3u
 /* k1_tuning: the plan for k = 1 over the assembled matrix: CSR-Adaptive
  * (NULL = its defaults), or the column-sorted SCOO when its flags hold
  * MFP_TUNE_SCOO_COLSORT (two kernels, not bit-exact to a row order). */

@@ -244,6 +244,192 @@ static int spmm_bins_build(uint32_t rows, const uint32_t *rp_dev, spmm_bins_args
   return MFP_OK;
 }
 
+// ---- CSR-stream SpMM ----------------------------------------------------------------
+// A workgroup of STREAM_T lanes takes one row block of fill_row_blocks with
+// STREAM_W entries (whole rows: at most STREAM_W entries and STREAM_W + 1
+// rows, or one longer row alone).  The block's columns and values are staged
+// in LDS with coalesced loads.  Then groups of LPE lanes (the 16-byte pieces
+// of an X row: KV s / 16 of them) take EPG consecutive entries each, issue
+// all their X-row gathers together, and sum the row segments in registers in
+// entry order with fma.  A row inside one group is written by that group; a
+// row that crosses groups is finished by the group holding its last entry:
+// its own segment, then the segments of the earlier groups from the nearest
+// back, added through LDS.  A longer row: group g sums entries g, g + NG, ...
+// (four in flight), then a halving tree over the groups.  Every lane is
+// busy whatever the row lengths, and a group's gathers do not wait on one
+// another, where CSR-vector leaves lanes idle on short rows and each row is
+// one chain of dependent loads.
+constexpr uint32_t STREAM_T = 256; // STREAM_W (kernels.h): entries per row block
+
+typedef float sv_f4 __attribute__((ext_vector_type(4)));
+typedef double sv_d2 __attribute__((ext_vector_type(2)));
+template <typename T> struct vec16b;
+template <> struct vec16b<float> {
+  using V = sv_f4;
+  static constexpr uint32_t N = 4;
+};
+template <> struct vec16b<double> {
+  using V = sv_d2;
+  static constexpr uint32_t N = 2;
+};
+
+template <typename T, int KV>
+__global__ __launch_bounds__(STREAM_T) void k_spmm_stream(const uint32_t *__restrict__ blocks,
+                                                          const uint32_t *__restrict__ rp,
+                                                          const uint32_t *__restrict__ ci, const T *__restrict__ a,
+                                                          const T *__restrict__ X, T *__restrict__ Y)
+{
+  using V = typename vec16b<T>::V;
+  constexpr uint32_t N = vec16b<T>::N, LPE = KV / N, NG = STREAM_T / LPE, EPG = STREAM_W / NG;
+  static_assert(KV % N == 0 && LPE >= 1 && STREAM_W % NG == 0, "X rows must be whole 16-byte pieces");
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
+  __shared__ uint32_t s_col[STREAM_W];
+  __shared__ T s_val[STREAM_W];
+  __shared__ uint32_t s_row[STREAM_W];
+  __shared__ uint32_t s_rp[STREAM_W + 2];
+  __shared__ V s_seg[STREAM_T];
+  const uint32_t r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
+  const uint32_t tid = threadIdx.x, g = tid / LPE, l = tid % LPE;
+  const uint32_t e0 = rp[r0], e1 = rp[r1];
+  const uint32_t n = e1 - e0, nr = r1 - r0;
+  const V *__restrict__ Xv = reinterpret_cast<const V *>(X);
+  V *__restrict__ Yv = reinterpret_cast<V *>(Y);
+  const V zero = (V)0;
+  if (n > STREAM_W) {
+    // one row (the partition puts a longer row alone)
+    V acc = zero;
+    uint32_t e = e0 + g;
+    for (; e + 3 * NG < e1; e += 4 * NG) {
+      T v[4];
+      uint32_t c[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; u++) {
+        v[u] = ldg_stream(a + e + u * NG);
+        c[u] = ldg_stream(ci + e + u * NG);
+      }
+      V xv[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; u++) xv[u] = Xv[(uint64_t)c[u] * LPE + l];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; u++)
+#pragma unroll
+        for (uint32_t i = 0; i < N; i++) acc[i] = fma_t(v[u], xv[u][i], acc[i]);
+    }
+    for (; e < e1; e += NG) {
+      const T v = ldg_stream(a + e);
+      const V xv = Xv[(uint64_t)ldg_stream(ci + e) * LPE + l];
+#pragma unroll
+      for (uint32_t i = 0; i < N; i++) acc[i] = fma_t(v, xv[i], acc[i]);
+    }
+    s_seg[tid] = acc;
+    __syncthreads();
+    for (uint32_t h = NG / 2; h >= 1; h >>= 1) {
+      if (g < h) s_seg[tid] = s_seg[tid] + s_seg[tid + h * LPE];
+      __syncthreads();
+    }
+    if (g == 0) Yv[(uint64_t)r0 * LPE + l] = s_seg[l];
+    return;
+  }
+  for (uint32_t i = tid; i <= nr; i += STREAM_T) s_rp[i] = rp[r0 + i] - e0;
+  for (uint32_t i = tid; i < n; i += STREAM_T) {
+    s_col[i] = ldg_stream(ci + e0 + i);
+    s_val[i] = ldg_stream(a + e0 + i);
+  }
+  __syncthreads();
+  for (uint32_t r = tid; r < nr; r += STREAM_T) {
+    const uint32_t b = s_rp[r], z = s_rp[r + 1];
+    if (b == z) {
+      for (uint32_t k = 0; k < LPE; k++) Yv[(uint64_t)(r0 + r) * LPE + k] = zero; // an empty row
+    }
+    for (uint32_t e = b; e < z; e++) s_row[e] = r;
+  }
+  __syncthreads();
+  const uint32_t base = g * EPG;
+  V xv[EPG];
+  T v[EPG];
+  uint32_t rr[EPG];
+#pragma unroll
+  for (uint32_t j = 0; j < EPG; j++) {
+    const uint32_t e = base + j;
+    const bool ok = e < n;
+    rr[j] = ok ? s_row[e] : NONE;
+    v[j] = ok ? s_val[e] : (T)0;
+    xv[j] = Xv[(uint64_t)(ok ? s_col[e] : 0u) * LPE + l]; // past the block: X row 0, never summed into a row
+  }
+  V acc = zero, head = zero;
+  uint32_t head_row = NONE, cur = rr[0];
+#pragma unroll
+  for (uint32_t j = 0; j < EPG; j++) {
+    if (rr[j] != cur) {
+      // row cur ends inside this group
+      if (s_rp[cur] < base) {
+        head = acc;
+        head_row = cur;
+      } else {
+        Yv[(uint64_t)(r0 + cur) * LPE + l] = acc;
+      }
+      acc = zero;
+      cur = rr[j];
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < N; i++) acc[i] = fma_t(v[j], xv[j][i], acc[i]);
+  }
+  if (cur != NONE) {
+    if (s_rp[cur + 1] > base + EPG) {
+      s_seg[tid] = acc; // row cur goes on in the next group
+    } else if (s_rp[cur] < base) {
+      head = acc;
+      head_row = cur;
+    } else {
+      Yv[(uint64_t)(r0 + cur) * LPE + l] = acc;
+    }
+  }
+  __syncthreads();
+  if (head_row != NONE) {
+    const uint32_t ga = s_rp[head_row] / EPG;
+    V sum = head;
+    for (uint32_t q = g; q-- > ga;) sum = sum + s_seg[q * LPE + l];
+    Yv[(uint64_t)(r0 + head_row) * LPE + l] = sum;
+  }
+}
+
+template <typename T>
+static int spmm_stream_t(uint32_t k, uint32_t nblocks, const uint32_t *blocks, const uint32_t *rp,
+                         const uint32_t *ci, const T *a, const T *X, T *Y, hipStream_t s)
+{
+  if (nblocks == 0) return MFP_OK;
+  switch (k) {
+  case 2:
+    if constexpr (sizeof(T) == 8) {
+      k_spmm_stream<T, 2><<<nblocks, STREAM_T, 0, s>>>(blocks, rp, ci, a, X, Y);
+      break;
+    }
+    [[fallthrough]];
+  case 1: set_error("the CSR-stream SpMM needs k values to fill whole 16-byte pieces (k >= 4 for f32, >= 2 for f64)");
+    return MFP_ERR_ARG;
+  case 4: k_spmm_stream<T, 4><<<nblocks, STREAM_T, 0, s>>>(blocks, rp, ci, a, X, Y); break;
+  case 8: k_spmm_stream<T, 8><<<nblocks, STREAM_T, 0, s>>>(blocks, rp, ci, a, X, Y); break;
+  case 16: k_spmm_stream<T, 16><<<nblocks, STREAM_T, 0, s>>>(blocks, rp, ci, a, X, Y); break;
+  default: set_error("k must be 1, 2, 4, 8 or 16"); return MFP_ERR_ARG;
+  }
+  return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
+}
+
+int launch_spmm_stream(int dtype, uint32_t k, uint32_t nblocks, const uint32_t *blocks, const uint32_t *rp,
+                       const uint32_t *ci, const void *a, const void *X, void *Y, hipStream_t s)
+{
+  if (((uintptr_t)X % 16) != 0 || ((uintptr_t)Y % 16) != 0) {
+    set_error("the CSR-stream SpMM needs 16-byte aligned X and Y");
+    return MFP_ERR_ARG;
+  }
+  if (dtype == MFP_F32)
+    return spmm_stream_t<float>(k, nblocks, blocks, rp, ci, (const float *)a, (const float *)X, (float *)Y, s);
+  if (dtype == MFP_F64)
+    return spmm_stream_t<double>(k, nblocks, blocks, rp, ci, (const double *)a, (const double *)X, (double *)Y, s);
+  set_error("SpMM values must be f32 or f64");
+  return MFP_ERR_ARG;
+}
+
 // ---- batch assembly ---------------------------------------------------------------
 __global__ void k_offset_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t n,
                              uint32_t add)
@@ -275,6 +461,8 @@ struct mfp_batch_s {
   uint32_t lanes = 8;          // k > 1; 0: lanes per row from its length (binned)
   spmm_bins_args bins{};       // lanes == 0
   uint32_t *bin_list = nullptr;
+  uint32_t *stream_blocks = nullptr; // lanes == MFP_BATCH_STREAM: row blocks of STREAM_W entries
+  uint32_t nstream = 0;
   mfp_plan cs[5] = {};         // lanes == 1: column-sorted SCOO plan for k = 2 << (i - 1)
 };
 
@@ -339,6 +527,8 @@ static int batch_launch(mfp_batch b, uint32_t k, const void *X, void *Y, hipStre
   if (b->lanes == 0)
     return launch_spmm_binned(m->dtype, k, b->bins, b->bin_list, m->rows, m->row_ptr, m->columns, m->values, X,
                               Y, s);
+  if (b->lanes == MFP_BATCH_STREAM)
+    return launch_spmm_stream(m->dtype, k, b->nstream, b->stream_blocks, m->row_ptr, m->columns, m->values, X, Y, s);
   if (b->lanes == 1) {
     const uint32_t i = log2_k(k);
     if (i > 4 || !b->cs[i]) {
@@ -364,7 +554,13 @@ int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream sv)
     }
     return launch_scoo_cs_spmm(p->dtype, k, p->m->rows, p->cs, p->scoo_threads, X, Y, as_stream(sv));
   }
-  MFP_ARG(p->format == MFP_CSR_VECTOR, "SpMM with k > 1 runs on CSR-vector and column-sorted SCOO plans");
+  if (p->format == MFP_CSR_ADAPTIVE && p->stream_blocks) {
+    const mfp_matrix m = p->m;
+    return launch_spmm_stream(p->dtype, k, p->nstream, p->stream_blocks, m->row_ptr, m->columns, m->values, X, Y,
+                              as_stream(sv));
+  }
+  MFP_ARG(p->format == MFP_CSR_VECTOR,
+          "SpMM with k > 1 runs on CSR-vector, CSR-Adaptive (f32 / f64) and column-sorted SCOO plans");
   MFP_ARG(p->dtype == MFP_F32 || p->dtype == MFP_F64, "SpMM values must be f32 or f64");
   const mfp_matrix m = p->m;
   return launch_spmm_vector(p->dtype, k, p->t.lanes, m->rows, m->row_ptr, m->columns, m->values, X, Y,
@@ -375,7 +571,7 @@ int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint3
                      const mfp_tuning *k1_tuning, mfp_stream sv)
 {
   MFP_ARG(out && ms && count > 0, "null pointer or empty batch");
-  MFP_ARG(lanes <= 1 || (lanes <= 64 && (lanes & (lanes - 1)) == 0), "lanes");
+  MFP_ARG(lanes <= 1 || lanes == MFP_BATCH_STREAM || (lanes <= 64 && (lanes & (lanes - 1)) == 0), "lanes");
   MFP_CHECK(ensure_init());
   hipStream_t s = as_stream(sv);
   const int dt = ms[0]->dtype;
@@ -431,6 +627,8 @@ int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint3
     st = mfp_plan_create(&b->k1, m, f, k1_tuning, sv);
   }
   if (st == MFP_OK && lanes == 0) st = spmm_bins_build(m->rows, m->row_ptr, &b->bins, &b->bin_list, s);
+  if (st == MFP_OK && lanes == MFP_BATCH_STREAM)
+    st = row_blocks_device(m->rows, m->row_ptr, STREAM_W, &b->stream_blocks, &b->nstream, s);
   if (st != MFP_OK) {
     mfp_batch_destroy(b);
     return st;
@@ -539,6 +737,7 @@ int mfp_batch_destroy(mfp_batch b)
   for (mfp_plan p : b->cs) mfp_plan_destroy(p);
   mfp_matrix_destroy(b->big);
   dfree(b->bin_list);
+  dfree(b->stream_blocks);
   delete b;
   return MFP_OK;
 }

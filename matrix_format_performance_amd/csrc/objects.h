@@ -27,6 +27,10 @@ struct mfp_plan_s {
   uint32_t slices = 0, ss = 0, L = 0, scoo_threads = 512;
   uint32_t *sp = nullptr;
   mfp::adaptive_layout ad;
+  // CSR-Adaptive plans over f32 / f64 values: row blocks of STREAM_W entries
+  // for the CSR-stream SpMM (mfp_plan_spmm with k > 1)
+  uint32_t *stream_blocks = nullptr;
+  uint32_t nstream = 0;
   uint64_t device_bytes = 0;
   // 16-bit-value ELL: columns as u16 offsets from base[row] when off16
   bool off16 = false;

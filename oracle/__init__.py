@@ -59,6 +59,7 @@ def _load() -> C.CDLL:
         "orc_spmv_csr_lanes": (None, [i32, u32, u32, _u32p, _u32p, _vp, _vp, _vp]),
         "orc_spmv_ell_seq": (None, [i32, u32, u32, _u32p, _vp, _vp, _vp]),
         "orc_spmv_coo_seq": (None, [i32, u64, _u32p, _u32p, _vp, _vp, _vp]),
+        "orc_spmm_stream_col": (i32, [i32, u32, u32, u32, _u32p, _u32p, _vp, _vp, _vp]),
         "orc_spmv_csr_adaptive": (None, [i32, u32, _u32p, _u32p, _vp, _vp, _vp, u32, u32, u32, u32]),
         "orc_spmv_csr_f64": (None, [i32, u32, _u32p, _u32p, _vp, _vp, _f64p, _vp]),
         "orc_spmv_csr_multi_thread": (dbl, [i32, u32, u32, _u32p, _u32p, _vp, _vp, _vp]),
@@ -254,6 +255,26 @@ def spmv_coo_seq(rows, coo_rows, coo_cols, coo_data, x, y=None):
     if coo_data.size:
         _lib.orc_spmv_coo_seq(_dt(coo_data), coo_data.size, coo_rows, coo_cols, _ptr(coo_data), _ptr(x), _ptr(y))
     return y
+
+
+def spmm_stream(row_ptr, columns, data, X, k, W=512, threads=256):
+    """Order of the library's CSR-stream SpMM (batch.hip k_spmm_stream) for
+    X (cols x k, row-major): groups = threads / (lanes per X row), a lane
+    holding 16 bytes of the row; each column is summed alike."""
+    row_ptr, columns, data = _c(row_ptr, np.uint32), _c(columns, np.uint32), _c(data)
+    X = np.asarray(X, data.dtype).reshape(-1, k)
+    groups = threads // (k * data.dtype.itemsize // 16)
+    rows = row_ptr.size - 1
+    Y = np.zeros((rows, k), data.dtype)
+    cols_ = columns if columns.size else np.zeros(1, np.uint32)
+    vals_ = data if data.size else np.zeros(1, data.dtype)
+    for j in range(k):
+        x = np.ascontiguousarray(X[:, j])
+        y = np.zeros(rows, data.dtype)
+        if _lib.orc_spmm_stream_col(_dt(data), W, groups, rows, row_ptr, cols_, _ptr(vals_), _ptr(x), _ptr(y)):
+            raise ValueError("bad W / groups")
+        Y[:, j] = y
+    return Y
 
 
 def spmv_hybrid_host_coo(rows, width, ell_cols, ell_data, coo_rows, coo_cols, coo_data, x):

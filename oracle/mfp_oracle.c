@@ -435,6 +435,81 @@ void orc_spmv_coo_seq(int dtype, uint64_t n, const uint32_t *coo_rows,
   }
 }
 
+/* The summation order of this library's CSR-stream SpMM (one column of X;
+ * every column is summed alike).  Row blocks from orc_fill_row_blocks(W).  A
+ * block with more than W entries is one row: group g (of `groups`) adds
+ * entries g, g + groups, ... with fma, then a halving tree
+ * p[g] += p[g + h], h = groups/2 .. 1, gives p[0].  Otherwise entry e of the
+ * block belongs to group e / (W / groups); a row's segment in each group is
+ * an fma sum from 0 in entry order; a row inside one group is that sum, and a
+ * row across groups ga..gb is seg[gb] + seg[gb-1] + ... + seg[ga], added in
+ * that order.  An empty row is 0. */
+#define STREAM_BODY(T, FMA)                                                        \
+  do {                                                                             \
+    const T *a = data, *xx = x;                                                    \
+    T *yy = y;                                                                     \
+    for (uint32_t b = 0; b < nb; b++) {                                            \
+      const uint32_t r0 = blocks[b], r1 = blocks[b + 1];                           \
+      const uint32_t e0 = row_ptr[r0], n = row_ptr[r1] - e0;                       \
+      if (n > W) {                                                                 \
+        for (uint32_t g = 0; g < groups; g++) {                                    \
+          T p = 0;                                                                 \
+          for (uint32_t e = g; e < n; e += groups)                                 \
+            p = FMA(a[e0 + e], xx[columns[e0 + e]], p);                            \
+          part[g] = (double)p;                                                     \
+        }                                                                          \
+        for (uint32_t h = groups / 2; h >= 1; h /= 2)                              \
+          for (uint32_t g = 0; g < h; g++) part[g] = (double)((T)part[g] + (T)part[g + h]); \
+        yy[r0] = (T)part[0];                                                       \
+        continue;                                                                  \
+      }                                                                            \
+      for (uint32_t r = r0; r < r1; r++) {                                         \
+        const uint32_t lo = row_ptr[r] - e0, hi = row_ptr[r + 1] - e0;            \
+        if (lo == hi) {                                                            \
+          yy[r] = 0;                                                               \
+          continue;                                                                \
+        }                                                                          \
+        const uint32_t ga = lo / epg, gb = (hi - 1) / epg;                         \
+        T sum = 0;                                                                 \
+        for (uint32_t g = gb + 1; g-- > ga;) {                                     \
+          const uint32_t s0 = g * epg > lo ? g * epg : lo;                         \
+          const uint32_t s1 = (g + 1) * epg < hi ? (g + 1) * epg : hi;             \
+          T seg = 0;                                                               \
+          for (uint32_t e = s0; e < s1; e++)                                       \
+            seg = FMA(a[e0 + e], xx[columns[e0 + e]], seg);                        \
+          sum = g == gb ? seg : sum + seg;                                         \
+        }                                                                          \
+        yy[r] = sum;                                                               \
+      }                                                                            \
+    }                                                                              \
+  } while (0)
+
+int orc_spmm_stream_col(int dtype, uint32_t W, uint32_t groups, uint32_t rows,
+                        const uint32_t *row_ptr, const uint32_t *columns,
+                        const void *data, const void *x, void *y)
+{
+  if (W == 0 || groups == 0 || W % groups != 0 || (groups & (groups - 1)) != 0)
+    return -1;
+  const uint32_t epg = W / groups;
+  const uint32_t nb = orc_fill_row_blocks(0, rows, row_ptr, NULL, W);
+  uint32_t *blocks = malloc(((size_t)nb + 1) * sizeof(uint32_t));
+  double *part = malloc((size_t)groups * sizeof(double));
+  if (!blocks || !part) {
+    free(blocks);
+    free(part);
+    return -1;
+  }
+  orc_fill_row_blocks(1, rows, row_ptr, blocks, W);
+  if (dtype == ORC_F32)
+    STREAM_BODY(float, fmaf);
+  else
+    STREAM_BODY(double, fma);
+  free(blocks);
+  free(part);
+  return 0;
+}
+#undef STREAM_BODY
+
 void orc_spmv_ell_seq(int dtype, uint32_t rows, uint32_t width,
                       const uint32_t *ell_cols, const void *ell_data,
                       const void *x, void *y)

@@ -121,6 +121,17 @@ void orc_spmv_coo_seq(int dtype, uint64_t n, const uint32_t *coo_rows,
                       const uint32_t *coo_cols, const void *coo_data,
                       const void *x, void *y);
 
+/* Summation order of this library's CSR-stream SpMM, one column of X
+ * (batch.hip k_spmm_stream): row blocks of fill_row_blocks(W); a block with
+ * more than W entries is one row summed by `groups` lane groups (group g:
+ * entries g, g + groups, ... with fma) and a halving tree; otherwise a row's
+ * segment in each group of W / groups consecutive entries is an fma sum, and
+ * a row across groups ga..gb is seg[gb] + seg[gb-1] + ... + seg[ga].  Returns
+ * 0, or -1 for a bad W / groups. */
+int orc_spmm_stream_col(int dtype, uint32_t W, uint32_t groups, uint32_t rows,
+                        const uint32_t *row_ptr, const uint32_t *columns,
+                        const void *data, const void *x, void *y);
+
 /* ELL: thread per row, sequential over the padded width with fma
  * (gpu/gpu_matrix_multiplier.cu:450-471). */
 void orc_spmv_ell_seq(int dtype, uint32_t rows, uint32_t width,

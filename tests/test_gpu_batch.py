@@ -123,6 +123,73 @@ def test_batch_binned_lanes_per_row(gpu, k, dtype):
     assert np.array_equal(Y, ref)
 
 
+@pytest.mark.parametrize("dtype,k", [("float32", 4), ("float32", 8), ("float32", 16),
+                                      ("float64", 2), ("float64", 4), ("float64", 16)])
+def test_batch_stream_spmm_bit_exact(gpu, k, dtype):
+    # CSR-stream SpMM (row blocks of 512 entries staged in LDS, groups of
+    # lanes over consecutive entries, rows across groups finished through
+    # LDS): every column equal to the oracle's emulation of that order,
+    # including rows longer than a block and empty rows
+    announce(f"batch stream k={k} dtype={dtype}")
+    members = _batch_members(gpu, 24, dtype, seed=40 + k)
+    members.append((np.zeros(4, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.dtype(dtype))))  # 3 empty rows
+    dms = [gpu.DeviceMatrix.from_csr(*m, cols=max(m[0].size - 1, 1)) for m in members]
+    batch = gpu.Batch(dms, lanes=gpu._lib.MFP_BATCH_STREAM)
+    for d in dms:
+        d.destroy()
+    rp, ci, va = batch.matrix_arrays()
+    lens = np.diff(rp.astype(np.int64))
+    assert lens.max() > 512 and (lens == 0).any()
+    X = np.random.default_rng(7 + k).uniform(-1, 1, (batch.cols, k)).astype(dtype)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray(batch.rows * k, np.dtype(dtype))
+    Yd.fill(np.nan)
+    batch.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Y = Yd.to_host().reshape(-1, k)
+    assert np.array_equal(Y, oracle.spmm_stream(rp, ci, va, X, k))
+    g = batch.graph(k, Xd, Yd)
+    for _ in range(2):
+        Yd.fill(np.nan)
+        g.launch()
+        gpu.synchronize()
+        assert np.array_equal(Yd.to_host().reshape(-1, k), Y)
+    g.destroy()
+    batch.destroy()
+
+
+@pytest.mark.parametrize("dtype,k", [("float32", 4), ("float32", 16), ("float64", 2), ("float64", 8)])
+@pytest.mark.parametrize("name", ["stencil3d_20", "powerlaw_20k", "edge_rows"])
+def test_plan_stream_spmm_bit_exact(gpu, name, k, dtype):
+    # mfp_plan_spmm on a CSR-Adaptive plan runs the CSR-stream SpMM
+    rp, ci, va, cols = case(name, dtype)
+    announce(f"plan stream spmm case={name} k={k} dtype={dtype}")
+    X = np.random.default_rng(k).uniform(-1, 1, (cols, k)).astype(dtype)
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "csr_adaptive")
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray((rp.size - 1) * k, np.dtype(dtype))
+    Yd.fill(np.nan)
+    plan.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    assert np.array_equal(Yd.to_host().reshape(-1, k), oracle.spmm_stream(rp, ci, va, X, k))
+    plan.destroy()
+    A.destroy()
+
+
+def test_batch_stream_spmm_refuses_narrow_x(gpu):
+    members = _batch_members(gpu, 2, "float32", seed=3)
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    batch = gpu.Batch(dms, lanes=gpu._lib.MFP_BATCH_STREAM)
+    Xd = gpu.DeviceArray(batch.cols * 2, np.float32)
+    Yd = gpu.DeviceArray(batch.rows * 2, np.float32)
+    with pytest.raises(gpu.MfpError, match="16-byte"):
+        batch.spmm(2, Xd, Yd)  # 2 floats = 8 bytes per X row
+    for d in dms:
+        d.destroy()
+    batch.destroy()
+
+
 def test_batch_graph_replays_match_eager(gpu):
     announce("batch graph replay")
     members = _batch_members(gpu, 16, "float32", seed=7)

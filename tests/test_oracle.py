@@ -375,6 +375,58 @@ def test_hybrid_host_coo_integer_closed_form(w):
         assert cd.size == 0 and np.array_equal(y, oracle.spmv_ell(200, w, ec, ed, x))
 
 
+@pytest.mark.parametrize("k,dt", [(4, np.float32), (16, np.float32), (2, np.float64), (16, np.float64)])
+def test_spmm_stream_order_against_fp64_and_integers(k, dt):
+    # random rows with empty ones and rows longer than a 512-entry block
+    rng = np.random.default_rng(31)
+    rows, cols = 700, 900
+    lens = rng.choice([0, 1, 3, 7, 8, 9, 30, 200, 511, 512, 513, 1500], rows)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    ci = rng.integers(0, cols, int(rp[-1])).astype(np.uint32)
+    va = rng.uniform(-1, 1, ci.size).astype(dt)
+    X = rng.uniform(-1, 1, (cols, k)).astype(dt)
+    Y = oracle.spmm_stream(rp, ci, va, X, k)
+    rr = np.repeat(np.arange(rows), lens)
+    prod = va.astype(np.float64)[:, None] * X.astype(np.float64)[ci]
+    want = np.zeros((rows, k)); np.add.at(want, rr, prod)
+    absd = np.zeros((rows, k)); np.add.at(absd, rr, np.abs(prod))
+    eps = np.finfo(dt).eps
+    assert np.all(np.abs(Y - want) <= 2 * lens[:, None] * eps * absd + 1e-300)
+    # integer values: every order is exact
+    vi = rng.integers(-3, 4, ci.size).astype(dt)
+    Xi = rng.integers(-3, 4, (cols, k)).astype(dt)
+    Yi = oracle.spmm_stream(rp, ci, vi, Xi, k)
+    wi = np.zeros((rows, k)); np.add.at(wi, rr, vi.astype(np.float64)[:, None] * Xi.astype(np.float64)[ci])
+    assert np.array_equal(Yi, wi.astype(dt))
+
+
+def test_spmm_stream_cross_group_order_known_answer():
+    # k = 16 FP32: 64 groups of 8 entries.  One row of 24 entries spans groups
+    # 0, 1 and 2 with segment sums -2^24, 2^24 and 1: the row is
+    # seg2 + seg1 + seg0 = (1 + 2^24) - 2^24 = 0 in float (1 + 2^24 rounds to
+    # 2^24), where entry order would give 1
+    v = np.zeros(24, np.float32)
+    v[0], v[8], v[16] = -2.0 ** 24, 2.0 ** 24, 1.0
+    rp = np.array([0, 24], np.uint32)
+    ci = np.zeros(24, np.uint32)
+    Y = oracle.spmm_stream(rp, ci, v, np.ones((1, 16), np.float32), 16)
+    assert np.all(Y == 0.0)
+    assert oracle.spmv_seq(rp, ci, v, np.ones(1, np.float32))[0] == 1.0
+
+
+def test_spmm_stream_long_row_tree_known_answer():
+    # a 576-entry row (longer than a block): group g adds entries g, g + 64,
+    # ...; with group sums p0 = 1, p1 = 2^24, p33 = -2^24 the halving tree
+    # gives p1 + p33 = 0 first, then p0 + 0 = 1; adding the groups in order
+    # would give (1 + 2^24) - 2^24 = 0
+    v = np.zeros(576, np.float32)
+    v[0], v[1], v[33] = 1.0, 2.0 ** 24, -2.0 ** 24
+    rp = np.array([0, 576], np.uint32)
+    ci = np.zeros(576, np.uint32)
+    Y = oracle.spmm_stream(rp, ci, v, np.ones((1, 16), np.float32), 16)
+    assert np.all(Y == 1.0)
+
+
 def test_multi_thread_matches_single_thread():
     r, c, v = _random_csr(1000, 1000, 30, seed=17)
     rp, ci, va = oracle.csr_from_triplets(1000, 1000, r, c, v, np.float32)
