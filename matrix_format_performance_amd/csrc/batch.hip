@@ -247,10 +247,15 @@ static int spmm_bins_build(uint32_t rows, const uint32_t *rp_dev, spmm_bins_args
 // ---- CSR-stream SpMM ----------------------------------------------------------------
 // A workgroup of STREAM_T lanes takes one row block of fill_row_blocks with
 // STREAM_W entries (whole rows: at most STREAM_W entries and STREAM_W + 1
-// rows, or one longer row alone).  The block's columns and values are staged
-// in LDS with coalesced loads.  Then groups of LPE lanes (the 16-byte pieces
-// of an X row: KV s / 16 of them) take EPG consecutive entries each, issue
-// all their X-row gathers together, and sum the row segments in registers in
+// rows, or one longer row alone).  Each block is described by a pair
+// (first row, first entry), so the block's bounds arrive in one load.  The
+// block's columns, values and row offsets are staged in LDS, all loads
+// issued before the first wait; each entry's row is found by writing the
+// row number at each row's first entry and taking a running maximum over
+// the entries (a wave scan plus the wave totals).  Then groups of LPE lanes
+// (the 16-byte pieces of an X row: KV s / 16 of them) take EPG consecutive
+// entries each, read their columns from LDS as whole vectors, issue all
+// their X-row gathers together, and sum the row segments in registers in
 // entry order with fma.  A row inside one group is written by that group; a
 // row that crosses groups is finished by the group holding its last entry:
 // its own segment, then the segments of the earlier groups from the nearest
@@ -258,8 +263,10 @@ static int spmm_bins_build(uint32_t rows, const uint32_t *rp_dev, spmm_bins_args
 // (four in flight), then a halving tree over the groups.  Every lane is
 // busy whatever the row lengths, and a group's gathers do not wait on one
 // another, where CSR-vector leaves lanes idle on short rows and each row is
-// one chain of dependent loads.
+// one chain of dependent loads.  Blocks are remapped so that each XCD works
+// on one contiguous range of them.
 constexpr uint32_t STREAM_T = 256; // STREAM_W (kernels.h): entries per row block
+static_assert(STREAM_W == 2 * STREAM_T, "the row scan gives each lane two entries");
 
 typedef float sv_f4 __attribute__((ext_vector_type(4)));
 typedef double sv_d2 __attribute__((ext_vector_type(2)));
@@ -273,8 +280,33 @@ template <> struct vec16b<double> {
   static constexpr uint32_t N = 2;
 };
 
-template <typename T, int KV>
-__global__ __launch_bounds__(STREAM_T) void k_spmm_stream(const uint32_t *__restrict__ blocks,
+// CNT consecutive LDS elements starting at a multiple of CNT, read as 16- or
+// 8-byte vectors.
+template <uint32_t CNT, typename E>
+__device__ __forceinline__ void lds_run(const E *p, E (&out)[CNT])
+{
+  constexpr uint32_t B = CNT * sizeof(E);
+  if constexpr (B % 16 == 0) {
+#pragma unroll
+    for (uint32_t i = 0; i < B / 16; i++) {
+      const uint4 w = reinterpret_cast<const uint4 *>(p)[i];
+      __builtin_memcpy(&out[i * (16 / sizeof(E))], &w, 16);
+    }
+  } else if constexpr (B % 8 == 0) {
+#pragma unroll
+    for (uint32_t i = 0; i < B / 8; i++) {
+      const uint2 w = reinterpret_cast<const uint2 *>(p)[i];
+      __builtin_memcpy(&out[i * (8 / sizeof(E))], &w, 8);
+    }
+  } else {
+#pragma unroll
+    for (uint32_t i = 0; i < CNT; i++) out[i] = p[i];
+  }
+}
+
+// OFF32: X is below 4 GiB, so a gather's address is X plus a 32-bit offset.
+template <typename T, int KV, bool OFF32>
+__global__ __launch_bounds__(STREAM_T, sizeof(T) == 4 ? 8 : 1) void k_spmm_stream(const uint2 *__restrict__ meta,
                                                           const uint32_t *__restrict__ rp,
                                                           const uint32_t *__restrict__ ci, const T *__restrict__ a,
                                                           const T *__restrict__ X, T *__restrict__ Y)
@@ -283,14 +315,16 @@ __global__ __launch_bounds__(STREAM_T) void k_spmm_stream(const uint32_t *__rest
   constexpr uint32_t N = vec16b<T>::N, LPE = KV / N, NG = STREAM_T / LPE, EPG = STREAM_W / NG;
   static_assert(KV % N == 0 && LPE >= 1 && STREAM_W % NG == 0, "X rows must be whole 16-byte pieces");
   constexpr uint32_t NONE = 0xFFFFFFFFu;
-  __shared__ uint32_t s_col[STREAM_W];
-  __shared__ T s_val[STREAM_W];
-  __shared__ uint32_t s_row[STREAM_W];
+  __shared__ __align__(16) uint32_t s_col[STREAM_W];
+  __shared__ __align__(16) T s_val[STREAM_W];
+  __shared__ __align__(16) uint32_t s_row[STREAM_W];
   __shared__ uint32_t s_rp[STREAM_W + 2];
   __shared__ V s_seg[STREAM_T];
-  const uint32_t r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
-  const uint32_t tid = threadIdx.x, g = tid / LPE, l = tid % LPE;
-  const uint32_t e0 = rp[r0], e1 = rp[r1];
+  __shared__ uint32_t s_wmax[STREAM_T / 64];
+  const uint32_t bi = xcd_block(blockIdx.x, gridDim.x);
+  const uint2 m0 = meta[bi], m1 = meta[bi + 1];
+  const uint32_t r0 = m0.x, r1 = m1.x, e0 = m0.y, e1 = m1.y;
+  const uint32_t tid = threadIdx.x, g = tid / LPE, l = tid % LPE, lane = tid & 63, wv = tid >> 6;
   const uint32_t n = e1 - e0, nr = r1 - r0;
   const V *__restrict__ Xv = reinterpret_cast<const V *>(X);
   V *__restrict__ Yv = reinterpret_cast<V *>(Y);
@@ -330,37 +364,86 @@ __global__ __launch_bounds__(STREAM_T) void k_spmm_stream(const uint32_t *__rest
     if (g == 0) Yv[(uint64_t)r0 * LPE + l] = s_seg[l];
     return;
   }
-  for (uint32_t i = tid; i <= nr; i += STREAM_T) s_rp[i] = rp[r0 + i] - e0;
-  for (uint32_t i = tid; i < n; i += STREAM_T) {
-    s_col[i] = ldg_stream(ci + e0 + i);
-    s_val[i] = ldg_stream(a + e0 + i);
+  {
+    // staging: entries tid and tid + STREAM_T, row offsets tid, + STREAM_T and + 2 STREAM_T (nr <= STREAM_W + 1)
+    const uint32_t i1 = tid + STREAM_T, i2 = tid + 2 * STREAM_T;
+    uint32_t c0 = 0, c1 = 0, p0 = 0, p1 = 0, p2 = 0;
+    T v0 = 0, v1 = 0;
+    if (tid < n) {
+      c0 = ldg_stream(ci + e0 + tid);
+      v0 = ldg_stream(a + e0 + tid);
+    }
+    if (i1 < n) {
+      c1 = ldg_stream(ci + e0 + i1);
+      v1 = ldg_stream(a + e0 + i1);
+    }
+    if (tid <= nr) p0 = rp[r0 + tid];
+    if (i1 <= nr) p1 = rp[r0 + i1];
+    if (i2 <= nr) p2 = rp[r0 + i2];
+    if (tid <= nr) s_rp[tid] = p0 - e0;
+    if (i1 <= nr) s_rp[i1] = p1 - e0;
+    if (i2 <= nr) s_rp[i2] = p2 - e0;
+    s_col[tid] = c0;
+    s_col[i1] = c1;
+    s_val[tid] = v0;
+    s_val[i1] = v1;
+    s_row[tid] = 0;
+    s_row[i1] = 0;
   }
   __syncthreads();
   for (uint32_t r = tid; r < nr; r += STREAM_T) {
     const uint32_t b = s_rp[r], z = s_rp[r + 1];
     if (b == z) {
       for (uint32_t k = 0; k < LPE; k++) Yv[(uint64_t)(r0 + r) * LPE + k] = zero; // an empty row
+    } else {
+      s_row[b] = r; // the row's first entry
     }
-    for (uint32_t e = b; e < z; e++) s_row[e] = r;
+  }
+  __syncthreads();
+  {
+    // entry -> row: running maximum over the entries; lane tid holds 2 tid and 2 tid + 1
+    const uint2 p = *reinterpret_cast<const uint2 *>(&s_row[2 * tid]);
+    const uint32_t hi = max(p.x, p.y);
+    uint32_t x = hi;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x = max(x, y);
+    }
+    if (lane == 63) s_wmax[wv] = x;
+    __syncthreads();
+    uint32_t ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0;
+    for (uint32_t i = 0; i < wv; i++) ex = max(ex, s_wmax[i]);
+    *reinterpret_cast<uint2 *>(&s_row[2 * tid]) = make_uint2(max(ex, p.x), max(ex, hi));
   }
   __syncthreads();
   const uint32_t base = g * EPG;
   V xv[EPG];
-  T v[EPG];
-  uint32_t rr[EPG];
+  {
+    uint32_t cc[EPG];
+    lds_run<EPG>(&s_col[base], cc);
 #pragma unroll
-  for (uint32_t j = 0; j < EPG; j++) {
-    const uint32_t e = base + j;
-    const bool ok = e < n;
-    rr[j] = ok ? s_row[e] : NONE;
-    v[j] = ok ? s_val[e] : (T)0;
-    xv[j] = Xv[(uint64_t)(ok ? s_col[e] : 0u) * LPE + l]; // past the block: X row 0, never summed into a row
+    for (uint32_t j = 0; j < EPG; j++) {
+      const uint32_t c = base + j < n ? cc[j] : 0u; // past the block: X row 0, never summed into a row
+      if constexpr (OFF32)
+        xv[j] = *reinterpret_cast<const V *>(reinterpret_cast<const char *>(X) + (c * (LPE * 16u) + l * 16u));
+      else
+        xv[j] = Xv[(uint64_t)c * LPE + l];
+    }
   }
+  uint32_t rr[EPG];
+  T v[EPG];
+  lds_run<EPG>(&s_row[base], rr);
+  lds_run<EPG>(&s_val[base], v);
   V acc = zero, head = zero;
-  uint32_t head_row = NONE, cur = rr[0];
+  uint32_t head_row = NONE, cur = base < n ? rr[0] : NONE;
 #pragma unroll
   for (uint32_t j = 0; j < EPG; j++) {
-    if (rr[j] != cur) {
+    const bool ok = base + j < n;
+    const uint32_t rj = ok ? rr[j] : NONE;
+    const T vj = ok ? v[j] : (T)0;
+    if (rj != cur) {
       // row cur ends inside this group
       if (s_rp[cur] < base) {
         head = acc;
@@ -369,10 +452,10 @@ __global__ __launch_bounds__(STREAM_T) void k_spmm_stream(const uint32_t *__rest
         Yv[(uint64_t)(r0 + cur) * LPE + l] = acc;
       }
       acc = zero;
-      cur = rr[j];
+      cur = rj;
     }
 #pragma unroll
-    for (uint32_t i = 0; i < N; i++) acc[i] = fma_t(v[j], xv[j][i], acc[i]);
+    for (uint32_t i = 0; i < N; i++) acc[i] = fma_t(vj, xv[j][i], acc[i]);
   }
   if (cur != NONE) {
     if (s_rp[cur + 1] > base + EPG) {
@@ -393,41 +476,91 @@ __global__ __launch_bounds__(STREAM_T) void k_spmm_stream(const uint32_t *__rest
   }
 }
 
+template <typename T, int KV>
+static void spmm_stream_launch(uint32_t nblocks, bool off32, const uint2 *meta, const uint32_t *rp,
+                               const uint32_t *ci, const T *a, const T *X, T *Y, hipStream_t s)
+{
+  if (off32)
+    k_spmm_stream<T, KV, true><<<nblocks, STREAM_T, 0, s>>>(meta, rp, ci, a, X, Y);
+  else
+    k_spmm_stream<T, KV, false><<<nblocks, STREAM_T, 0, s>>>(meta, rp, ci, a, X, Y);
+}
+
 template <typename T>
-static int spmm_stream_t(uint32_t k, uint32_t nblocks, const uint32_t *blocks, const uint32_t *rp,
+static int spmm_stream_t(uint32_t k, uint32_t cols, uint32_t nblocks, const uint2 *meta, const uint32_t *rp,
                          const uint32_t *ci, const T *a, const T *X, T *Y, hipStream_t s)
 {
   if (nblocks == 0) return MFP_OK;
+  const bool off32 = (uint64_t)cols * k * sizeof(T) <= 0xFFFFFFFFull;
   switch (k) {
   case 2:
     if constexpr (sizeof(T) == 8) {
-      k_spmm_stream<T, 2><<<nblocks, STREAM_T, 0, s>>>(blocks, rp, ci, a, X, Y);
+      spmm_stream_launch<T, 2>(nblocks, off32, meta, rp, ci, a, X, Y, s);
       break;
     }
     [[fallthrough]];
   case 1: set_error("the CSR-stream SpMM needs k values to fill whole 16-byte pieces (k >= 4 for f32, >= 2 for f64)");
     return MFP_ERR_ARG;
-  case 4: k_spmm_stream<T, 4><<<nblocks, STREAM_T, 0, s>>>(blocks, rp, ci, a, X, Y); break;
-  case 8: k_spmm_stream<T, 8><<<nblocks, STREAM_T, 0, s>>>(blocks, rp, ci, a, X, Y); break;
-  case 16: k_spmm_stream<T, 16><<<nblocks, STREAM_T, 0, s>>>(blocks, rp, ci, a, X, Y); break;
+  case 4: spmm_stream_launch<T, 4>(nblocks, off32, meta, rp, ci, a, X, Y, s); break;
+  case 8: spmm_stream_launch<T, 8>(nblocks, off32, meta, rp, ci, a, X, Y, s); break;
+  case 16: spmm_stream_launch<T, 16>(nblocks, off32, meta, rp, ci, a, X, Y, s); break;
   default: set_error("k must be 1, 2, 4, 8 or 16"); return MFP_ERR_ARG;
   }
   return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("kernel launch failed"), MFP_ERR_HIP);
 }
 
-int launch_spmm_stream(int dtype, uint32_t k, uint32_t nblocks, const uint32_t *blocks, const uint32_t *rp,
-                       const uint32_t *ci, const void *a, const void *X, void *Y, hipStream_t s)
+int launch_spmm_stream(int dtype, uint32_t k, uint32_t cols, uint32_t nblocks, const uint32_t *meta,
+                       const uint32_t *rp, const uint32_t *ci, const void *a, const void *X, void *Y, hipStream_t s)
 {
   if (((uintptr_t)X % 16) != 0 || ((uintptr_t)Y % 16) != 0) {
     set_error("the CSR-stream SpMM needs 16-byte aligned X and Y");
     return MFP_ERR_ARG;
   }
+  const uint2 *m = reinterpret_cast<const uint2 *>(meta);
   if (dtype == MFP_F32)
-    return spmm_stream_t<float>(k, nblocks, blocks, rp, ci, (const float *)a, (const float *)X, (float *)Y, s);
+    return spmm_stream_t<float>(k, cols, nblocks, m, rp, ci, (const float *)a, (const float *)X, (float *)Y, s);
   if (dtype == MFP_F64)
-    return spmm_stream_t<double>(k, nblocks, blocks, rp, ci, (const double *)a, (const double *)X, (double *)Y, s);
+    return spmm_stream_t<double>(k, cols, nblocks, m, rp, ci, (const double *)a, (const double *)X, (double *)Y, s);
   set_error("SpMM values must be f32 or f64");
   return MFP_ERR_ARG;
+}
+
+// Row blocks of STREAM_W entries as (first row, first entry) pairs: n + 1 of
+// them, the last one (rows, nnz).
+__global__ void k_stream_meta(uint32_t n, const uint32_t *__restrict__ blocks, const uint32_t *__restrict__ rp,
+                              uint32_t *__restrict__ meta)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const uint32_t r = blocks[i];
+  meta[2 * i] = r;
+  meta[2 * i + 1] = rp[r];
+}
+
+int stream_meta_device(uint32_t rows, const uint32_t *rp, uint32_t **meta, uint32_t *nblocks, hipStream_t s)
+{
+  *meta = nullptr;
+  *nblocks = 0;
+  uint32_t *blocks = nullptr, n = 0;
+  int st = row_blocks_device(rows, rp, STREAM_W, &blocks, &n, s);
+  if (st != MFP_OK) return st;
+  uint32_t *m = nullptr;
+  st = dmalloc((void **)&m, ((uint64_t)n + 1) * 8);
+  if (st == MFP_OK) {
+    k_stream_meta<<<(n + 1 + 255) / 256, 256, 0, s>>>(n, blocks, rp, m);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      set_error("row block pairs: kernel failed");
+      st = MFP_ERR_HIP;
+    }
+  }
+  dfree(blocks);
+  if (st != MFP_OK) {
+    dfree(m);
+    return st;
+  }
+  *meta = m;
+  *nblocks = n;
+  return MFP_OK;
 }
 
 // ---- batch assembly ---------------------------------------------------------------
@@ -461,7 +594,7 @@ struct mfp_batch_s {
   uint32_t lanes = 8;          // k > 1; 0: lanes per row from its length (binned)
   spmm_bins_args bins{};       // lanes == 0
   uint32_t *bin_list = nullptr;
-  uint32_t *stream_blocks = nullptr; // lanes == MFP_BATCH_STREAM: row blocks of STREAM_W entries
+  uint32_t *stream_meta = nullptr; // lanes == MFP_BATCH_STREAM: row blocks of STREAM_W entries, (row, entry) pairs
   uint32_t nstream = 0;
   mfp_plan cs[5] = {};         // lanes == 1: column-sorted SCOO plan for k = 2 << (i - 1)
 };
@@ -528,7 +661,8 @@ static int batch_launch(mfp_batch b, uint32_t k, const void *X, void *Y, hipStre
     return launch_spmm_binned(m->dtype, k, b->bins, b->bin_list, m->rows, m->row_ptr, m->columns, m->values, X,
                               Y, s);
   if (b->lanes == MFP_BATCH_STREAM)
-    return launch_spmm_stream(m->dtype, k, b->nstream, b->stream_blocks, m->row_ptr, m->columns, m->values, X, Y, s);
+    return launch_spmm_stream(m->dtype, k, m->cols, b->nstream, b->stream_meta, m->row_ptr, m->columns, m->values, X, Y,
+                              s);
   if (b->lanes == 1) {
     const uint32_t i = log2_k(k);
     if (i > 4 || !b->cs[i]) {
@@ -554,10 +688,10 @@ int mfp_plan_spmm(mfp_plan p, uint32_t k, const void *X, void *Y, mfp_stream sv)
     }
     return launch_scoo_cs_spmm(p->dtype, k, p->m->rows, p->cs, p->scoo_threads, X, Y, as_stream(sv));
   }
-  if (p->format == MFP_CSR_ADAPTIVE && p->stream_blocks) {
+  if (p->format == MFP_CSR_ADAPTIVE && p->stream_meta) {
     const mfp_matrix m = p->m;
-    return launch_spmm_stream(p->dtype, k, p->nstream, p->stream_blocks, m->row_ptr, m->columns, m->values, X, Y,
-                              as_stream(sv));
+    return launch_spmm_stream(p->dtype, k, m->cols, p->nstream, p->stream_meta, m->row_ptr, m->columns, m->values, X,
+                              Y, as_stream(sv));
   }
   MFP_ARG(p->format == MFP_CSR_VECTOR,
           "SpMM with k > 1 runs on CSR-vector, CSR-Adaptive (f32 / f64) and column-sorted SCOO plans");
@@ -628,7 +762,7 @@ int mfp_batch_create(mfp_batch *out, const mfp_matrix *ms, uint32_t count, uint3
   }
   if (st == MFP_OK && lanes == 0) st = spmm_bins_build(m->rows, m->row_ptr, &b->bins, &b->bin_list, s);
   if (st == MFP_OK && lanes == MFP_BATCH_STREAM)
-    st = row_blocks_device(m->rows, m->row_ptr, STREAM_W, &b->stream_blocks, &b->nstream, s);
+    st = stream_meta_device(m->rows, m->row_ptr, &b->stream_meta, &b->nstream, s);
   if (st != MFP_OK) {
     mfp_batch_destroy(b);
     return st;
@@ -737,7 +871,7 @@ int mfp_batch_destroy(mfp_batch b)
   for (mfp_plan p : b->cs) mfp_plan_destroy(p);
   mfp_matrix_destroy(b->big);
   dfree(b->bin_list);
-  dfree(b->stream_blocks);
+  dfree(b->stream_meta);
   delete b;
   return MFP_OK;
 }

@@ -70,10 +70,12 @@ void free_adaptive_layout(adaptive_layout *l);
 int launch_spmm_vector(int dtype, uint32_t k, uint32_t lanes, uint32_t rows, const uint32_t *rp,
                        const uint32_t *ci, const void *a, const void *X, void *Y, bool persistent,
                        hipStream_t s);
-// CSR-stream SpMM over row blocks of STREAM_W entries (fill_row_blocks, batch.hip)
+// CSR-stream SpMM over row blocks of STREAM_W entries (fill_row_blocks, batch.hip).
+// meta holds nblocks + 1 (first row, first entry) pairs from stream_meta_device.
 constexpr uint32_t STREAM_W = 512;
-int launch_spmm_stream(int dtype, uint32_t k, uint32_t nblocks, const uint32_t *blocks, const uint32_t *rp,
-                       const uint32_t *ci, const void *a, const void *X, void *Y, hipStream_t s);
+int stream_meta_device(uint32_t rows, const uint32_t *rp, uint32_t **meta, uint32_t *nblocks, hipStream_t s);
+int launch_spmm_stream(int dtype, uint32_t k, uint32_t cols, uint32_t nblocks, const uint32_t *meta,
+                       const uint32_t *rp, const uint32_t *ci, const void *a, const void *X, void *Y, hipStream_t s);
 int offset_copy_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t add, hipStream_t s);
 
 int csr_from_triplets_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz,

@@ -28,8 +28,9 @@ struct mfp_plan_s {
   uint32_t *sp = nullptr;
   mfp::adaptive_layout ad;
   // CSR-Adaptive plans over f32 / f64 values: row blocks of STREAM_W entries
-  // for the CSR-stream SpMM (mfp_plan_spmm with k > 1)
-  uint32_t *stream_blocks = nullptr;
+  // as (first row, first entry) pairs for the CSR-stream SpMM (mfp_plan_spmm
+  // with k > 1)
+  uint32_t *stream_meta = nullptr;
   uint32_t nstream = 0;
   uint64_t device_bytes = 0;
   // 16-bit-value ELL: columns as u16 offsets from base[row] when off16

@@ -154,7 +154,7 @@ static int plan_free_arrays(mfp_plan p)
   scoo_cs_free(&p->cs);
   lp_binned_free(&p->lpb);
   free_adaptive_layout(&p->ad);
-  dfree(p->stream_blocks);
+  dfree(p->stream_meta);
   return MFP_OK;
 }
 
@@ -520,9 +520,9 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
     st = build_adaptive_layout(m->dtype, m->rows, m->row_ptr, t.nnz_per_wg, t.long_row, t.chunk,
                                &p->ad, s, !(t.flags & MFP_TUNE_STATIC_PARTIALS));
     if (st != MFP_OK) return fail(st);
-    st = row_blocks_device(m->rows, m->row_ptr, STREAM_W, &p->stream_blocks, &p->nstream, s);
+    st = stream_meta_device(m->rows, m->row_ptr, &p->stream_meta, &p->nstream, s);
     if (st != MFP_OK) return fail(st);
-    p->device_bytes = ((uint64_t)p->ad.nblocks + 1) * 4 + ((uint64_t)p->nstream + 1) * 4 +
+    p->device_bytes = ((uint64_t)p->ad.nblocks + 1) * 4 + ((uint64_t)p->nstream + 1) * 8 +
                       (uint64_t)p->ad.nchunks * (16 + (p->ad.partials ? vs : 0));
     break;
   case MFP_ELL:

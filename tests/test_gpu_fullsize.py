@@ -20,7 +20,11 @@ too.  On top of that, size-independent properties:
 - the 512-matrix batch (about 20M rows, 430M nnz) for k = 1, 4 and 16:
   sampled members' blocks against the oracle on the member alone, a
   checksum of checksums over all rows, and HIP-graph replays equal to the
-  eager launch.
+  eager launch;
+- the same batch through the CSR-stream SpMM at k = 16: row ranges cut at
+  row-block boundaries bit-exact against the oracle's emulation of that
+  order on the range alone (the greedy partition restarted at a block
+  boundary gives the same blocks), and the checksum of checksums.
 """
 import functools
 
@@ -275,6 +279,53 @@ def test_batch512_member_blocks_and_checksum(gpu):
                         for j in range(512))
             assert abs(float(Y[:, c].astype(np.float64).sum()) - want) <= 1e-5 * bound, (k, c)
     batch.destroy()
+
+
+def test_batch512_stream_spmm_k16(gpu):
+    members = gpu.engine.batch_workload()
+    announce("full size batch512 stream spmm k=16")
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    del members
+    batch = gpu.Batch(dms, lanes=gpu._lib.MFP_BATCH_STREAM)
+    for d in dms:
+        d.destroy()
+    rp, ci, va = batch.matrix_arrays()
+    k = 16
+    X = np.random.default_rng(16).uniform(-1, 1, (batch.cols, k)).astype(np.float32)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray(batch.rows * k, np.float32)
+    Yd.fill(np.nan)
+    batch.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Y = Yd.to_host().reshape(-1, k)
+    g = batch.graph(k, Xd, Yd)
+    Yd.fill(np.nan)
+    g.launch()
+    gpu.synchronize()
+    assert np.array_equal(Yd.to_host().reshape(-1, k), Y)
+    g.destroy()
+    Xd.free()
+    Yd.free()
+    batch.destroy()
+    assert not np.isnan(Y).any()
+    blocks = oracle.fill_row_blocks(rp, 512).astype(np.int64)
+    nb = blocks.size - 1
+    lens = np.diff(rp.astype(np.int64))
+    longest = int(np.searchsorted(blocks, int(np.argmax(lens)), side="right")) - 1
+    starts = sorted({0, nb // 3, nb // 2, 2 * nb // 3, max(nb - 48, 0), max(longest - 8, 0)})
+    for b0 in starts:
+        b1 = min(b0 + 48, nb)
+        ra, rb = int(blocks[b0]), int(blocks[b1])
+        ea, eb = int(rp[ra]), int(rp[rb])
+        sub_rp = (rp[ra:rb + 1].astype(np.int64) - ea).astype(np.uint32)
+        want = oracle.spmm_stream(sub_rp, ci[ea:eb], va[ea:eb], X, k)
+        assert np.array_equal(Y[ra:rb], want), (b0, b1)
+    colsum = np.bincount(ci, weights=va.astype(np.float64), minlength=X.shape[0])
+    abscol = np.bincount(ci, weights=np.abs(va.astype(np.float64)), minlength=X.shape[0])
+    for c in range(k):
+        want = float(colsum @ X[:, c].astype(np.float64))
+        bound = float(abscol @ np.abs(X[:, c].astype(np.float64)))
+        assert abs(float(Y[:, c].astype(np.float64).sum()) - want) <= 1e-5 * bound, c
 
 
 @pytest.mark.parametrize("vt", ["bf16", "f16"])
