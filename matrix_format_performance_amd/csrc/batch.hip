@@ -259,8 +259,9 @@ static int spmm_bins_build(uint32_t rows, const uint32_t *rp_dev, spmm_bins_args
 // entry order with fma.  A row inside one group is written by that group; a
 // row that crosses groups is finished by the group holding its last entry:
 // its own segment, then the segments of the earlier groups from the nearest
-// back, added through LDS.  A longer row: group g sums entries g, g + NG, ...
-// (four in flight), then a halving tree over the groups.  Every lane is
+// back, added through LDS.  A longer row goes through LDS in chunks of
+// STREAM_W entries, group g summing its EPG entries of each chunk in turn,
+// then a halving tree over the groups.  Every lane is
 // busy whatever the row lengths, and a group's gathers do not wait on one
 // another, where CSR-vector leaves lanes idle on short rows and each row is
 // one chain of dependent loads.  Blocks are remapped so that each XCD works
@@ -304,9 +305,28 @@ __device__ __forceinline__ void lds_run(const E *p, E (&out)[CNT])
   }
 }
 
-// OFF32: X is below 4 GiB, so a gather's address is X plus a 32-bit offset.
+// Piece l (16 bytes) of X row c; OFF32: X is below 4 GiB, so the address is
+// X plus a 32-bit offset.
+template <typename V, uint32_t LPE, bool OFF32, typename T>
+__device__ __forceinline__ V stream_gather(const T *__restrict__ X, uint32_t c, uint32_t l)
+{
+  if constexpr (OFF32)
+    return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(X) + (c * (LPE * 16u) + l * 16u));
+  else
+    return reinterpret_cast<const V *>(X)[(uint64_t)c * LPE + l];
+}
+
+// Workgroups per CU that the register budget is set for: 8 (the wave limit)
+// for every f32 form and the narrow f64 ones; wider f64 X rows hold more
+// registers per gather.
+template <typename T, int KV>
+constexpr int stream_min_wg()
+{
+  return sizeof(T) == 4 || KV <= 4 ? 8 : (KV == 8 ? 5 : 4);
+}
+
 template <typename T, int KV, bool OFF32>
-__global__ __launch_bounds__(STREAM_T, sizeof(T) == 4 ? 8 : 1) void k_spmm_stream(const uint2 *__restrict__ meta,
+__global__ __launch_bounds__(STREAM_T, (stream_min_wg<T, KV>())) void k_spmm_stream(const uint2 *__restrict__ meta,
                                                           const uint32_t *__restrict__ rp,
                                                           const uint32_t *__restrict__ ci, const T *__restrict__ a,
                                                           const T *__restrict__ X, T *__restrict__ Y)
@@ -326,34 +346,60 @@ __global__ __launch_bounds__(STREAM_T, sizeof(T) == 4 ? 8 : 1) void k_spmm_strea
   const uint32_t r0 = m0.x, r1 = m1.x, e0 = m0.y, e1 = m1.y;
   const uint32_t tid = threadIdx.x, g = tid / LPE, l = tid % LPE, lane = tid & 63, wv = tid >> 6;
   const uint32_t n = e1 - e0, nr = r1 - r0;
-  const V *__restrict__ Xv = reinterpret_cast<const V *>(X);
   V *__restrict__ Yv = reinterpret_cast<V *>(Y);
   const V zero = (V)0;
   if (n > STREAM_W) {
-    // one row (the partition puts a longer row alone)
+    // one row (the partition puts a longer row alone), in chunks of
+    // STREAM_W entries staged in LDS with the next chunk's loads in flight
+    // during the current chunk's gathers.  Group g sums entries
+    // g EPG .. g EPG + EPG - 1 of every chunk, chunk after chunk, with fma;
+    // then a halving tree over the groups.
     V acc = zero;
-    uint32_t e = e0 + g;
-    for (; e + 3 * NG < e1; e += 4 * NG) {
-      T v[4];
-      uint32_t c[4];
-#pragma unroll
-      for (uint32_t u = 0; u < 4; u++) {
-        v[u] = ldg_stream(a + e + u * NG);
-        c[u] = ldg_stream(ci + e + u * NG);
-      }
-      V xv[4];
-#pragma unroll
-      for (uint32_t u = 0; u < 4; u++) xv[u] = Xv[(uint64_t)c[u] * LPE + l];
-#pragma unroll
-      for (uint32_t u = 0; u < 4; u++)
-#pragma unroll
-        for (uint32_t i = 0; i < N; i++) acc[i] = fma_t(v[u], xv[u][i], acc[i]);
+    const uint32_t i1 = tid + STREAM_T, base = g * EPG;
+    uint32_t c0 = 0, c1 = 0;
+    T v0 = 0, v1 = 0;
+    if (tid < n) {
+      c0 = ldg_stream(ci + e0 + tid);
+      v0 = ldg_stream(a + e0 + tid);
     }
-    for (; e < e1; e += NG) {
-      const T v = ldg_stream(a + e);
-      const V xv = Xv[(uint64_t)ldg_stream(ci + e) * LPE + l];
+    if (i1 < n) {
+      c1 = ldg_stream(ci + e0 + i1);
+      v1 = ldg_stream(a + e0 + i1);
+    }
+    for (uint32_t cb = 0; cb < n; cb += STREAM_W) {
+      s_col[tid] = c0;
+      s_col[i1] = c1;
+      s_val[tid] = v0;
+      s_val[i1] = v1;
+      __syncthreads();
+      const uint32_t rem = n - cb; // entries from this chunk's start to the row's end
+      if (rem > STREAM_W) {
+        const uint32_t nx = e0 + cb + STREAM_W, left = rem - STREAM_W;
+        if (tid < left) {
+          c0 = ldg_stream(ci + nx + tid);
+          v0 = ldg_stream(a + nx + tid);
+        }
+        if (i1 < left) {
+          c1 = ldg_stream(ci + nx + i1);
+          v1 = ldg_stream(a + nx + i1);
+        }
+      }
+      V xv[EPG];
+      {
+        uint32_t cc[EPG];
+        lds_run<EPG>(&s_col[base], cc);
 #pragma unroll
-      for (uint32_t i = 0; i < N; i++) acc[i] = fma_t(v, xv[i], acc[i]);
+        for (uint32_t j = 0; j < EPG; j++) xv[j] = stream_gather<V, LPE, OFF32>(X, base + j < rem ? cc[j] : 0u, l);
+      }
+      T vv[EPG];
+      lds_run<EPG>(&s_val[base], vv);
+#pragma unroll
+      for (uint32_t j = 0; j < EPG; j++)
+        if (base + j < rem) {
+#pragma unroll
+          for (uint32_t i = 0; i < N; i++) acc[i] = fma_t(vv[j], xv[j][i], acc[i]);
+        }
+      __syncthreads();
     }
     s_seg[tid] = acc;
     __syncthreads();
@@ -424,13 +470,8 @@ __global__ __launch_bounds__(STREAM_T, sizeof(T) == 4 ? 8 : 1) void k_spmm_strea
     uint32_t cc[EPG];
     lds_run<EPG>(&s_col[base], cc);
 #pragma unroll
-    for (uint32_t j = 0; j < EPG; j++) {
-      const uint32_t c = base + j < n ? cc[j] : 0u; // past the block: X row 0, never summed into a row
-      if constexpr (OFF32)
-        xv[j] = *reinterpret_cast<const V *>(reinterpret_cast<const char *>(X) + (c * (LPE * 16u) + l * 16u));
-      else
-        xv[j] = Xv[(uint64_t)c * LPE + l];
-    }
+    for (uint32_t j = 0; j < EPG; j++) // past the block: X row 0, never summed into a row
+      xv[j] = stream_gather<V, LPE, OFF32>(X, base + j < n ? cc[j] : 0u, l);
   }
   uint32_t rr[EPG];
   T v[EPG];

@@ -437,8 +437,9 @@ void orc_spmv_coo_seq(int dtype, uint64_t n, const uint32_t *coo_rows,
 
 /* The summation order of this library's CSR-stream SpMM (one column of X;
  * every column is summed alike).  Row blocks from orc_fill_row_blocks(W).  A
- * block with more than W entries is one row: group g (of `groups`) adds
- * entries g, g + groups, ... with fma, then a halving tree
+ * block with more than W entries is one row, taken in chunks of W entries:
+ * group g (of `groups`) adds entries c W + g (W / groups) + j, j < W / groups,
+ * of chunk c = 0, 1, ... in turn with fma, then a halving tree
  * p[g] += p[g + h], h = groups/2 .. 1, gives p[0].  Otherwise entry e of the
  * block belongs to group e / (W / groups); a row's segment in each group is
  * an fma sum from 0 in entry order; a row inside one group is that sum, and a
@@ -454,8 +455,11 @@ void orc_spmv_coo_seq(int dtype, uint64_t n, const uint32_t *coo_rows,
       if (n > W) {                                                                 \
         for (uint32_t g = 0; g < groups; g++) {                                    \
           T p = 0;                                                                 \
-          for (uint32_t e = g; e < n; e += groups)                                 \
-            p = FMA(a[e0 + e], xx[columns[e0 + e]], p);                            \
+          for (uint64_t cb = 0; cb < n; cb += W)                                   \
+            for (uint32_t j = 0; j < epg; j++) {                                   \
+              const uint64_t e = cb + (uint64_t)g * epg + j;                       \
+              if (e < n) p = FMA(a[e0 + e], xx[columns[e0 + e]], p);               \
+            }                                                                      \
           part[g] = (double)p;                                                     \
         }                                                                          \
         for (uint32_t h = groups / 2; h >= 1; h /= 2)                              \

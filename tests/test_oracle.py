@@ -415,16 +415,31 @@ def test_spmm_stream_cross_group_order_known_answer():
 
 
 def test_spmm_stream_long_row_tree_known_answer():
-    # a 576-entry row (longer than a block): group g adds entries g, g + 64,
-    # ...; with group sums p0 = 1, p1 = 2^24, p33 = -2^24 the halving tree
-    # gives p1 + p33 = 0 first, then p0 + 0 = 1; adding the groups in order
-    # would give (1 + 2^24) - 2^24 = 0
+    # a 576-entry row (longer than a block) at k = 16 FP32: 64 groups of 8
+    # entries per 512-entry chunk, so group g adds entries 8g .. 8g + 7, then
+    # 512 + 8g .. 512 + 8g + 7.  Group sums p0 = 1 (entry 0), p1 = 2^24
+    # (entry 8) and p33 = -2^24 (entry 264 = 8 * 33): the halving
+    # tree adds p1 + p33 = 0 first, then p0 + 0 = 1; the groups in order would
+    # give (1 + 2^24) - 2^24 = 0
     v = np.zeros(576, np.float32)
-    v[0], v[1], v[33] = 1.0, 2.0 ** 24, -2.0 ** 24
+    v[0], v[8], v[264] = 1.0, 2.0 ** 24, -2.0 ** 24
     rp = np.array([0, 576], np.uint32)
     ci = np.zeros(576, np.uint32)
     Y = oracle.spmm_stream(rp, ci, v, np.ones((1, 16), np.float32), 16)
     assert np.all(Y == 1.0)
+
+
+def test_spmm_stream_long_row_chunk_order_known_answer():
+    # within a group the chunks follow each other: group 0 of a 1,100-entry
+    # row adds entry 0 (2^24), entry 512 (1) and entry 1,024 (-2^24) in that
+    # order, (2^24 + 1) - 2^24 = 0 in float, where the order 0, 1,024, 512
+    # would give 1; every other entry is 0
+    v = np.zeros(1100, np.float32)
+    v[0], v[512], v[1024] = 2.0 ** 24, 1.0, -2.0 ** 24
+    rp = np.array([0, 1100], np.uint32)
+    ci = np.zeros(1100, np.uint32)
+    Y = oracle.spmm_stream(rp, ci, v, np.ones((1, 16), np.float32), 16)
+    assert np.all(Y == 0.0)
 
 
 def test_multi_thread_matches_single_thread():
