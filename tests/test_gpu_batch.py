@@ -177,6 +177,42 @@ def test_plan_stream_spmm_bit_exact(gpu, name, k, dtype):
     A.destroy()
 
 
+def test_plan_stream_spmm_x_above_4gib(gpu):
+    # X of 70M rows x 16 floats (4.48 GB): past 4 GiB the CSR-stream gathers
+    # take 64-bit offsets.  Most columns sit near the top of X, where a 32-bit
+    # byte offset would wrap to another X row, and X[c, j] = c % 251 + j tells
+    # the rows apart.  Integer values make every summation order exact.
+    rows, cols, k = 2048, 70_000_000, 16
+    announce(f"plan stream spmm X above 4 GiB: cols={cols} k={k}")
+    rng = np.random.default_rng(12)
+    lens = rng.integers(0, 40, rows)
+    lens[7] = 1500  # a row longer than a block
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    ci = rng.integers(cols - 3_000_000, cols, int(rp[-1])).astype(np.uint32)
+    ci[::5] = rng.integers(0, 1000, ci[::5].size)
+    va = rng.integers(-3, 4, ci.size).astype(np.float32)
+    xc = (np.arange(cols, dtype=np.uint32) % 251).astype(np.float32)
+    X = xc[:, None] + np.arange(k, dtype=np.float32)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    del X
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "csr_adaptive")
+    Yd = gpu.DeviceArray(rows * k, np.float32)
+    Yd.fill(np.nan)
+    plan.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Y = Yd.to_host().reshape(rows, k)
+    plan.destroy()
+    A.destroy()
+    Xd.free()
+    Yd.free()
+    rr = np.repeat(np.arange(rows), lens)
+    base = np.bincount(rr, weights=va.astype(np.float64) * xc[ci], minlength=rows)
+    vsum = np.bincount(rr, weights=va.astype(np.float64), minlength=rows)
+    want = base[:, None] + vsum[:, None] * np.arange(k)
+    assert np.array_equal(Y.astype(np.float64), want)
+
+
 def test_batch_stream_spmm_refuses_narrow_x(gpu):
     members = _batch_members(gpu, 2, "float32", seed=3)
     dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
