@@ -233,8 +233,8 @@ def test_batch512_member_blocks_and_checksum(gpu):
     announce(f"full size batch512 rows={sum(m[0].size - 1 for m in members)} "
              f"nnz={sum(m[1].size for m in members)}")
     dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
-    batch = gpu.Batch(dms, lanes=32, flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=16384,
-                      scoo_pieces=2, scoo_threads=1024)
+    batch = gpu.Batch(dms, lanes=32, flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=8192,
+                      scoo_pieces=1, scoo_threads=1024)
     for d in dms:
         d.destroy()
     r_off, c_off = batch.row_offsets, batch.col_offsets
