@@ -648,8 +648,9 @@ static uint32_t log2_k(uint32_t k)
 }
 
 // lanes == 1: the column-sorted plan for k columns, built on first use (never
-// inside a graph capture): the largest slice (up to 16384 rows) whose R k
-// double accumulators fit in LDS, about two items per band, 1024 threads.
+// inside a graph capture): the largest slice (a power of two, up to 16384
+// rows) whose R k double accumulators fill at most half of the LDS, so that
+// two workgroups share a CU, but never below 1024 rows; 1024 threads.
 static int batch_cs_plan(mfp_batch b, uint32_t k, mfp_stream sv)
 {
   const uint32_t i = log2_k(k);
@@ -661,8 +662,12 @@ static int batch_cs_plan(mfp_batch b, uint32_t k, mfp_stream sv)
   mfp_tuning t;
   MFP_CHECK(mfp_tuning_default(b->big, MFP_SCOO, &t));
   t.flags = MFP_TUNE_SCOO_COLSORT;
+  // On the 512-matrix batch the half-LDS slice beats the full-LDS one at
+  // k = 2 (920 against 1,027 us), 4 (1,538 against 1,664) and 8 (3,077
+  // against 3,238); at k = 16, 1024 rows (128 KB) beat 512 (6,526 against
+  // 6,787 us).
   t.scoo_slice_size = 16384;
-  while ((uint64_t)t.scoo_slice_size * k * 8 > 163840) t.scoo_slice_size /= 2;
+  while (t.scoo_slice_size > 1024 && (uint64_t)t.scoo_slice_size * k * 8 > 81920) t.scoo_slice_size /= 2;
   // one piece per slice: a slice of the block-diagonal batch seldom spans
   // more than one column band, so most slices write Y directly and skip the
   // partial round trip of R k values per slot (k = 4: 1.70 against 1.80 ms

@@ -21,6 +21,8 @@ too.  On top of that, size-independent properties:
   sampled members' blocks against the oracle on the member alone, a
   checksum of checksums over all rows, and HIP-graph replays equal to the
   eager launch;
+- the same batch through the column-sorted SpMM at k = 4: sampled members
+  per column within tolerance, eager and replayed, and the checksum;
 - the same batch through the CSR-stream SpMM at k = 16: row ranges cut at
   row-block boundaries bit-exact against the oracle's emulation of that
   order on the range alone (the greedy partition restarted at a block
@@ -279,6 +281,48 @@ def test_batch512_member_blocks_and_checksum(gpu):
                         for j in range(512))
             assert abs(float(Y[:, c].astype(np.float64).sum()) - want) <= 1e-5 * bound, (k, c)
     batch.destroy()
+
+
+def test_batch512_colsort_spmm_k4(gpu):
+    members = gpu.engine.batch_workload()
+    announce("full size batch512 column-sorted spmm k=4")
+    dms = [gpu.DeviceMatrix.from_csr(*m) for m in members]
+    batch = gpu.Batch(dms, lanes=1)
+    for d in dms:
+        d.destroy()
+    r_off, c_off = batch.row_offsets, batch.col_offsets
+    k = 4
+    X = np.random.default_rng(4).uniform(-1, 1, (batch.cols, k)).astype(np.float32)
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray(batch.rows * k, np.float32)
+    Yd.fill(np.nan)
+    batch.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Ys = [Yd.to_host().reshape(-1, k)]
+    g = batch.graph(k, Xd, Yd)
+    Yd.fill(np.nan)
+    g.launch()
+    gpu.synchronize()
+    Ys.append(Yd.to_host().reshape(-1, k))
+    g.destroy()
+    Xd.free()
+    Yd.free()
+    batch.destroy()
+    sample = [0, 1, 2, 3, 4, 5, 6, 7, int(np.argmax([m[1].size for m in members]))]
+    for Y in Ys:  # the eager launch, then the graph replay
+        assert not np.isnan(Y).any()
+        for j in sample:
+            rp, ci, va = members[j]
+            Yj = Y[r_off[j]:r_off[j + 1]]
+            for c in range(k):
+                _tol(np.ascontiguousarray(Yj[:, c]), rp, ci, va, np.ascontiguousarray(X[c_off[j]:c_off[j + 1], c]))
+    colsum = [np.bincount(ci, weights=va.astype(np.float64), minlength=rp.size - 1) for rp, ci, va in members]
+    abscol = [np.bincount(ci, weights=np.abs(va.astype(np.float64)), minlength=rp.size - 1)
+              for rp, ci, va in members]
+    for c in range(k):
+        want = sum(float(colsum[j] @ X[c_off[j]:c_off[j + 1], c].astype(np.float64)) for j in range(512))
+        bound = sum(float(abscol[j] @ np.abs(X[c_off[j]:c_off[j + 1], c].astype(np.float64))) for j in range(512))
+        assert abs(float(Ys[0][:, c].astype(np.float64).sum()) - want) <= 1e-5 * bound, c
 
 
 def test_batch512_stream_spmm_k16(gpu):
