@@ -174,8 +174,9 @@ enum {
   MFP_TUNE_IS_DEFAULT = 2,  /* set by the autotuner on each format's default candidate; no effect */
   /* SCOO with each slice's entries sorted by column and split into work
    * items (about scoo_pieces per slice; 0 = about two per CU over the whole
-   * matrix), never across a column band of 2^(32 - log2 scoo_slice_size)
-   * columns; scoo_slice_size rows (a power of two, at most 16384: 8 bytes
+   * matrix), none spanning 2^(32 - log2 scoo_slice_size) columns or more
+   * (a wider slice is cut at the multiples of that width); scoo_slice_size
+   * rows (a power of two, at most 16384: 8 bytes
    * of LDS per row) accumulate in LDS, and rows much longer than a slice's
    * mean share get slices of their own.  No reference counterpart (the
    * reference's SCOO keeps row order, scoo_matrix_class.cpp:102). */
@@ -185,7 +186,13 @@ enum {
    * binned 16-bit CSR): such a plan runs through mfp_plan_spmv_noalloc, and
    * mfp_plan_spmv / mfp_plan_spmm refuse it (MFP_ERR_STATE) when it has
    * partial results. */
-  MFP_TUNE_STATIC_PARTIALS = 8
+  MFP_TUNE_STATIC_PARTIALS = 8,
+  /* With MFP_TUNE_SCOO_COLSORT: no long-row slices and one work item per
+   * slice (more only where a slice's columns span a band or more), so
+   * scoo_pieces is ignored.  A layout whose every slice is one item has no
+   * partial slots and runs as one kernel.  Meant for many small matrices
+   * in one block-diagonal batch, where slices are many and narrow. */
+  MFP_TUNE_SCOO_WHOLE_SLICES = 16
 };
 
 typedef struct mfp_plan_s *mfp_plan;

@@ -673,6 +673,10 @@ static int batch_cs_plan(mfp_batch b, uint32_t k, mfp_stream sv)
   // partial round trip of R k values per slot (k = 4: 1.70 against 1.80 ms
   // with two pieces; the slices still number thousands for 256 CUs)
   t.scoo_pieces = 1;
+  // k <= 4: no long-row slices and one item per slice, so no partial slots
+  // and one kernel (k = 4: 1,484 against 1,523 us with long-row slices and
+  // the finish; k = 2 even; k = 8 and 16 were 1-2% slower)
+  if (k <= 4) t.flags |= MFP_TUNE_SCOO_WHOLE_SLICES;
   t.scoo_threads = 1024;
   return mfp_plan_create(&b->cs[i], b->big, MFP_SCOO, &t, sv);
 }

@@ -626,6 +626,16 @@ __global__ void k_slice_bounds(uint32_t S, uint64_t nnz, const uint32_t *__restr
   sp[s] = (uint32_t)lo;
 }
 
+// Column span of each slice: last column - first column of its sorted entries.
+__global__ void k_slice_span(uint32_t S, const uint32_t *__restrict__ sp, const uint32_t *__restrict__ col_sorted,
+                             uint32_t *__restrict__ span)
+{
+  const uint32_t sl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= S) return;
+  const uint32_t lo = sp[sl], hi = sp[sl + 1];
+  span[sl] = hi > lo ? col_sorted[hi - 1] - col_sorted[lo] : 0u;
+}
+
 __global__ void k_item_base(uint32_t NI, const uint4 *__restrict__ items, const uint32_t *__restrict__ col,
                             uint32_t *__restrict__ base, uint32_t *__restrict__ max_span)
 {
@@ -776,21 +786,34 @@ int scoo_cs_build_device(int dtype, uint32_t rows, uint32_t cols, uint64_t nnz, 
   if (hipMemcpyAsync(hsp.data(), out->sp, hsp.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return done(MFP_ERR_HIP);
-  // Column bands of width W = 2^(32 - rbits): an item inside one band packs
-  // (column - item base) and the local row into one u32.  Each non-empty
-  // band of a slice is cut into about E_band / item_entries equal pieces.
+  // Column bands of width W = 2^(32 - rbits): an item whose columns span
+  // less than W packs (column - item base) and the local row into one u32.
+  // A slice whose columns span less than W is one band; any other slice is
+  // cut at the multiples of W.  Each non-empty band of a slice is cut into
+  // about E_band / item_entries equal pieces.
   const uint64_t W = rbits ? (1ull << (32 - rbits)) : (1ull << 32);
   const uint32_t nb = (uint32_t)std::max<uint64_t>(1, ((uint64_t)cols + W - 1) / W);
   std::vector<uint32_t> hbb((size_t)S * (nb + 1));
+  std::vector<uint32_t> slice_span(S, 0u);
   if (S && nb > 1) {
-    uint32_t *bb = nullptr;
+    uint32_t *bb = nullptr, *sspan = nullptr;
     st = scratch((void **)&bb, hbb.size() * 4);
+    if (st == MFP_OK) st = scratch((void **)&sspan, (uint64_t)S * 4);
     if (st != MFP_OK) return done(st);
     k_band_bounds<<<ceil_div_u32(hbb.size(), 256), 256, 0, s>>>(S, nb, W, out->sp, cur[1], bb);
+    if (hipGetLastError() != hipSuccess) return done(MFP_ERR_HIP);
+    k_slice_span<<<ceil_div_u32(S, 256), 256, 0, s>>>(S, out->sp, cur[1], sspan);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(hbb.data(), bb, hbb.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(slice_span.data(), sspan, (size_t)S * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return done(MFP_ERR_HIP);
+    for (uint32_t sl = 0; sl < S; sl++) {
+      if (slice_span[sl] >= W) continue;
+      uint32_t *b = &hbb[(size_t)sl * (nb + 1)];
+      b[0] = hsp[sl]; // the whole slice in band 0, the other bands empty
+      for (uint32_t k = 1; k <= nb; k++) b[k] = hsp[sl + 1];
+    }
   } else {
     for (uint32_t sl = 0; sl < S; sl++) {
       hbb[(size_t)sl * 2] = hsp[sl];

@@ -451,7 +451,8 @@ int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *t)
 // 128) get their own slices: in an ordinary slice such a row would put many
 // lanes of every wave on one LDS address.  Work items take about
 // nnz / (2 CUs) entries each (at least 2048), or nnz / (slices * pieces)
-// when scoo_pieces is set.
+// when scoo_pieces is set.  MFP_TUNE_SCOO_WHOLE_SLICES: no long-row slices
+// and one item per slice.
 static int plan_create_scoo_cs(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t, hipStream_t s)
 {
   const uint32_t R = t.scoo_slice_size;
@@ -461,9 +462,12 @@ static int plan_create_scoo_cs(mfp_plan p, mfp_matrix_s *m, const mfp_tuning &t,
   }
   const uint64_t S = m->rows ? ((uint64_t)m->rows + R - 1) / R : 0u;
   const uint64_t per_slice = S ? m->nnz / S : 0;
-  const uint32_t T = (uint32_t)std::max<uint64_t>(256, per_slice / 128);
+  const bool whole = (t.flags & MFP_TUNE_SCOO_WHOLE_SLICES) != 0;
+  const uint32_t T = whole ? 0xFFFFFFFFu : (uint32_t)std::max<uint64_t>(256, per_slice / 128);
   uint64_t item;
-  if (t.scoo_pieces)
+  if (whole)
+    item = ~0ull; // one item per band of a slice
+  else if (t.scoo_pieces)
     item = std::max<uint64_t>(1, per_slice / t.scoo_pieces);
   else
     item = std::max<uint64_t>(2048, m->nnz / (2u * (uint64_t)std::max(1, props().cu_count)));

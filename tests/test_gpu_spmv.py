@@ -177,6 +177,63 @@ def test_scoo_colsort_wide_columns_cut_at_bands(gpu, dtype, P):
     _tol_check(yd.to_host(), rp, ci, va, x)
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("R", [256, 4096])
+@pytest.mark.parametrize("name", CASES)
+def test_scoo_colsort_whole_slices(gpu, name, dtype, R):
+    # MFP_TUNE_SCOO_WHOLE_SLICES: no long-row slices and, as every case has
+    # fewer columns than a band (2^(32 - log2 R) >= 2^20), one item per slice,
+    # so no partial slots; SpMV and a 4-column SpMM within tolerance
+    flags = gpu._lib.MFP_TUNE_SCOO_COLSORT | gpu._lib.MFP_TUNE_SCOO_WHOLE_SLICES
+    y, (rp, ci, va, x), info = _run(gpu, name, dtype, "scoo", flags=flags, scoo_slice_size=R, scoo_pieces=3)
+    rows = rp.size - 1
+    S = -(-rows // R)
+    assert info["scoo_long_rows"] == 0 and info["slices_count"] == S and info["lane_size"] == S
+    _tol_check(y, rp, ci, va, x)
+    k = 4
+    cols = x.size
+    X = np.random.default_rng(4).uniform(-1, 1, (cols, k)).astype(dtype)
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "scoo", flags=flags, scoo_slice_size=min(R, 4096))
+    Xd = gpu.DeviceArray.from_host(X.ravel())
+    Yd = gpu.DeviceArray(rows * k, np.dtype(dtype))
+    Yd.fill(np.nan)
+    plan.spmm(k, Xd, Yd)
+    gpu.synchronize()
+    Y = Yd.to_host().reshape(rows, k)
+    plan.destroy()
+    A.destroy()
+    for c in range(k):
+        _tol_check(np.ascontiguousarray(Y[:, c]), rp, ci, va, np.ascontiguousarray(X[:, c]))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_scoo_colsort_slice_across_a_band_edge_stays_whole(gpu, dtype):
+    # a banded matrix with 300,000 columns and 16384-row slices (bands of
+    # 262,144 columns): the slice holding rows near column 262,144 spans the
+    # band edge but fewer than 262,144 columns, so it stays one item
+    rp, ci, va = gpu.generate(gpu.GenSpec("uniform", values="uniform", rows=300000, nnz_lo=1, nnz_hi=8, seed=6,
+                                          bandwidth=0.01), np.dtype(dtype))
+    cols = rp.size - 1
+    R = 16384
+    S = -(-cols // R)
+    assert any(int(ci[rp[sl * R]:rp[min((sl + 1) * R, cols)]].min()) < 262144 <=
+               int(ci[rp[sl * R]:rp[min((sl + 1) * R, cols)]].max()) for sl in range(S))
+    announce(f"scoo colsort band edge dtype={dtype}")
+    flags = gpu._lib.MFP_TUNE_SCOO_COLSORT | gpu._lib.MFP_TUNE_SCOO_WHOLE_SLICES
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    plan = gpu.Plan(A, "scoo", flags=flags, scoo_slice_size=R)
+    info = plan.info()
+    assert info["scoo_packed"] == 1 and info["lane_size"] == S
+    x = gpu.generate_vector(3, cols, np.dtype(dtype))
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(cols, np.dtype(dtype))
+    yd.fill(np.nan)
+    plan.spmv(xd, yd)
+    gpu.synchronize()
+    _tol_check(yd.to_host(), rp, ci, va, x)
+
+
 def _colsort_run(gpu, rp, ci, va, cols, label, **tuning):
     announce(f"scoo colsort {label} rows={rp.size - 1} nnz={ci.size} tuning={tuning}")
     A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
