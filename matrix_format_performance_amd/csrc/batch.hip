@@ -326,10 +326,9 @@ constexpr int stream_min_wg()
 }
 
 template <typename T, int KV, bool OFF32>
-__global__ __launch_bounds__(STREAM_T, (stream_min_wg<T, KV>())) void k_spmm_stream(const uint2 *__restrict__ meta,
-                                                          const uint32_t *__restrict__ rp,
-                                                          const uint32_t *__restrict__ ci, const T *__restrict__ a,
-                                                          const T *__restrict__ X, T *__restrict__ Y)
+__global__ __launch_bounds__(STREAM_T, (stream_min_wg<T, KV>())) void k_spmm_stream(
+    const uint2 *__restrict__ meta, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+    const T *__restrict__ a, const T *__restrict__ X, T *__restrict__ Y)
 {
   using V = typename vec16b<T>::V;
   constexpr uint32_t N = vec16b<T>::N, LPE = KV / N, NG = STREAM_T / LPE, EPG = STREAM_W / NG;
