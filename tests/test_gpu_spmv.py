@@ -292,8 +292,9 @@ def test_scoo_colsort_long_rows_fill_several_slices(gpu, dtype, P):
     _tol_check(y, rp, ci, va, x)
 
 
+@pytest.mark.parametrize("whole", [False, True])
 @pytest.mark.parametrize("vt", ["bf16", "f16"])
-def test_scoo_colsort_16bit_values_in_the_last_band(gpu, vt):
+def test_scoo_colsort_16bit_values_in_the_last_band(gpu, vt, whole):
     rng = np.random.default_rng(13)
     rows, cols = 20000, 600000
     lens = rng.integers(0, 9, rows)
@@ -305,7 +306,10 @@ def test_scoo_colsort_16bit_values_in_the_last_band(gpu, vt):
     _, _, stored = A.download()
     vs = ((stored.astype(np.uint32) << 16).view(np.float32) if vt == "bf16"
           else stored.view(np.float16).astype(np.float32))
-    plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT, scoo_slice_size=16384, scoo_pieces=3)
+    flags = gpu._lib.MFP_TUNE_SCOO_COLSORT | (gpu._lib.MFP_TUNE_SCOO_WHOLE_SLICES if whole else 0)
+    plan = gpu.Plan(A, "scoo", flags=flags, scoo_slice_size=16384, scoo_pieces=3)
+    if whole:  # columns 524,288.. span less than a band of 262,144: one item per slice
+        assert plan.info()["lane_size"] == -(-rows // 16384)
     x = gpu.generate_vector(9, cols, np.float32)
     xd = gpu.DeviceArray.from_host(x)
     yd = gpu.DeviceArray(rows, np.float32)
