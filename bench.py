@@ -69,6 +69,8 @@ WORKLOADS = {
 def run_batch(mfp, args, timer):
     """One step = one replay of the captured persistent launch over all 512
     matrices (k right-hand sides); value = algorithmic bytes / mean step."""
+    copy_gbs = measured_copy_gbs(mfp, timer)
+    log(f"[bench] measured 16-byte copy: {copy_gbs:.1f} GB/s")
     t0 = time.time()
     members = mfp.engine.batch_workload()
     dms = [mfp.DeviceMatrix.from_csr(rp, ci, va) for rp, ci, va in members]
@@ -83,14 +85,14 @@ def run_batch(mfp, args, timer):
     modes = {"mean": mean_lanes, "binned": 0, "colsort": 1, "stream": mfp._lib.MFP_BATCH_STREAM}
     if mean_lanes >= 4:
         modes["half"] = mean_lanes // 2
-    if args.batch_lanes > 1:
+    if args.batch_lanes == mfp._lib.MFP_BATCH_STREAM:
+        modes = {"stream": mfp._lib.MFP_BATCH_STREAM}
+    elif args.batch_lanes > 1:
         modes = {f"lanes{args.batch_lanes}": args.batch_lanes}
     elif args.batch_lanes == 0:
         modes = {"binned": 0}
     elif args.batch_lanes == 1:
         modes = {"colsort": 1}
-    elif args.batch_lanes == mfp._lib.MFP_BATCH_STREAM:
-        modes = {"stream": mfp._lib.MFP_BATCH_STREAM}
     # k = 1: CSR-Adaptive (the first batch) or the column-sorted SCOO (the last)
     # (8,192-row slices, one item per slice: 2.6% faster than 16,384 rows in
     # two pieces over 33 interleaved flushed launches each)
@@ -167,7 +169,8 @@ def run_batch(mfp, args, timer):
         "config": {"workload": "batch512", "description": WORKLOADS["batch512"], "matrices": batch.count,
                    "plans": "per k in batch[k].plan (the fastest of those timed in ms_by_plan)",
                    "rows": rows, "cols": cols, "nnz": nnz, "launch": "one HIP graph replay per step (one persistent SpMM kernel, or the column-sorted SCOO main + finish kernels)",
-                   "cache": "Infinity Cache evicted before every step"},
+                   "cache": "Infinity Cache evicted before every step",
+                   "measured_copy_GB_per_s": round(copy_gbs, 1)},
         "pct_hbm_roofline": round(100 * value / HBM_PEAK_GBS, 2),
         "batch": res,
         "roofline": {"bound": "hbm", "achieved": value, "peak": HBM_PEAK_GBS, "unit": "GB/s",
