@@ -371,7 +371,7 @@ class Selector:
 
     @classmethod
     def default(cls) -> "Selector":
-        """The selector trained on this package's 2,000-matrix MI355X sweep
+        """The selector trained on this package's 2,400-matrix MI355X sweep
         (selector_mi355x.npz next to this file)."""
         return cls.load(DEFAULT_WEIGHTS)
 
