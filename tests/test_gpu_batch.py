@@ -356,7 +356,7 @@ def test_spmm_column_sorted_refuses_a_slice_too_large_for_k(gpu):
     gpu.synchronize()
 
 
-@pytest.mark.parametrize("k", [4, 16])
+@pytest.mark.parametrize("k", [2, 4, 8, 16])  # k <= 4: whole slices, one kernel; k > 4: long-row slices and a finish
 def test_batch_column_sorted_spmm(gpu, k):
     announce(f"batch column-sorted spmm k={k}")
     members = _batch_members(gpu, 16, "float32", seed=30 + k)
