@@ -9,7 +9,7 @@ trained matrix_format_performance_amd/selector_mi355x.npz.  Here:
   feature vectors; the picks (format + the tuning rules' tuning) must be
   within 5% of the exhaustive best on average, and beat the method without a
   selector (the one fixed format best on the training matrices);
-- every 16th held-out matrix (30) is generated again from its recorded
+- every 8th held-out matrix (60) is generated again from its recorded
   spec and seed: its size and its 64 features, computed on the GPU from
   integer counters, must equal the record exactly, and a fresh exhaustive
   timing of every candidate must keep the pick within 5% of the best on
@@ -30,7 +30,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SWEEPS = [os.path.join(ROOT, "results", n) for n in ("selector_c5_sweep.jsonl.gz", "selector_c5_sweep_big.jsonl.gz")]
 HOLD_EVERY = 5
-RETIME_EVERY = 16
+RETIME_EVERY = 8
 
 
 def _load(path):
