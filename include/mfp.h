@@ -192,7 +192,15 @@ enum {
    * scoo_pieces is ignored.  A layout whose every slice is one item has no
    * partial slots and runs as one kernel.  Meant for many small matrices
    * in one block-diagonal batch, where slices are many and narrow. */
-  MFP_TUNE_SCOO_WHOLE_SLICES = 16
+  MFP_TUNE_SCOO_WHOLE_SLICES = 16,
+  /* FP32 ELL plans: a layout of 64-row slices with 16-bit column offsets from
+   * each slice's first column, four slots to a word (16 bytes of values, 8 of
+   * offsets): 6 bytes per slot instead of 8, summed in the same slot order as
+   * the plain ELL.  Refused (MFP_ERR_ARG) for other value types and when a
+   * slice's columns span 65,536 or more.  Such a plan keeps no column-major
+   * arrays, so mfp_plan_array has none to give for it.  No reference
+   * counterpart (the reference's ELL is column-major, matrix_converter.cpp:170). */
+  MFP_TUNE_ELL_PACKED = 32
 };
 
 typedef struct mfp_plan_s *mfp_plan;

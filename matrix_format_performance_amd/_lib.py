@@ -62,6 +62,7 @@ MFP_TUNE_IS_DEFAULT = 2
 MFP_TUNE_SCOO_COLSORT = 4
 MFP_TUNE_STATIC_PARTIALS = 8
 MFP_TUNE_SCOO_WHOLE_SLICES = 16
+MFP_TUNE_ELL_PACKED = 32
 MFP_BATCH_STREAM = 3
 MFP_NOALLOC_SCRATCH_BYTES = 32 << 20
 

@@ -1,10 +1,10 @@
 // Autotuner: candidate (format, tuning) sets for a matrix and their timing.
 //
 // Swept parameters: lanes per row (CSR-vector, 2..64), CSR-Adaptive block
-// size, ELL width (longest row), the ELL/COO split of the hybrids (row-length
-// quantiles), the SCOO slice size (lane count filling a 32 KB LDS
-// accumulator) and column-sorted SCOO slices of 4096..16384 rows (pieces per
-// slice and workgroup size).  Every candidate is built as a plan and timed with an event
+// size, ELL width (longest row) and the packed FP32 ELL layout, the ELL/COO
+// split of the hybrids (row-length quantiles), the SCOO slice size (lane
+// count filling a 32 KB LDS accumulator) and column-sorted SCOO slices of
+// 4096..16384 rows (pieces per slice and workgroup size).  Every candidate is built as a plan and timed with an event
 // pair (median of reps, each after an Infinity-Cache flush when asked).
 #include "../../include/mfp.h"
 
@@ -135,6 +135,14 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
       }
       break;
     }
+    case MFP_ELL:
+      push(c, f, tagged);
+      if (v.dtype == MFP_F32) { // the packed layout (its plan refuses slices whose columns span 65,536 or more)
+        mfp_tuning tp = def;
+        tp.flags |= MFP_TUNE_ELL_PACKED;
+        push(c, f, tp);
+      }
+      break;
     default: push(c, f, tagged); break;
     }
   }

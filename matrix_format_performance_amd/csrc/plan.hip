@@ -151,6 +151,9 @@ static int plan_free_arrays(mfp_plan p)
   dfree(p->sp);
   dfree(p->lp_cols);
   dfree(p->lp_base);
+  dfree(p->pk_val);
+  dfree(p->pk_off);
+  dfree(p->pk_base);
   scoo_cs_free(&p->cs);
   lp_binned_free(&p->lpb);
   free_adaptive_layout(&p->ad);
@@ -446,6 +449,39 @@ int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *t)
   return MFP_OK;
 }
 
+// FP32 ELL with MFP_TUNE_ELL_PACKED (spmv.hip k_ell_pk): 64-row slices, each
+// with a base column (its rows' smallest first column) that every column of
+// the slice lies within 65,536 of; w rounded up to words of 4 slots.
+static int plan_create_ell_packed(mfp_plan p, mfp_matrix_s *m, uint32_t w, const mfp_tuning &t, hipStream_t s)
+{
+  const uint32_t stride = ell_lp_stride(m->rows);
+  const uint32_t w4 = (w + 3) / 4;
+  const uint64_t slots = (uint64_t)stride * 4u * w4;
+  const uint64_t need = slots * 6 + (uint64_t)stride / 64 * 4;
+  if (need > budget_of(t)) {
+    set_error("packed ELL needs " + std::to_string(need) + " bytes, above the budget");
+    return MFP_ERR_BUDGET;
+  }
+  uint32_t span = 0;
+  MFP_CHECK(dmalloc((void **)&p->pk_base, std::max<uint64_t>((uint64_t)stride / 64, 1) * 4));
+  if (m->rows) MFP_CHECK(block_base_build(m->rows, m->row_ptr, m->columns, p->pk_base, &span, s));
+  else MFP_CHECK(fill_u32(p->pk_base, 0, 1, s));
+  if (span >= 65536u) {
+    set_error("packed ELL: a 64-row slice spans " + std::to_string(span) + " columns, more than 16-bit offsets reach");
+    return MFP_ERR_ARG;
+  }
+  MFP_CHECK(dmalloc((void **)&p->pk_val, std::max<uint64_t>(slots, 1) * 4));
+  MFP_CHECK(dmalloc((void **)&p->pk_off, std::max<uint64_t>(slots, 1) * 2));
+  MFP_CHECK(ell_pk_build(m->rows, stride, m->row_ptr, m->columns, (const float *)m->values, w4, p->pk_base, p->pk_val,
+                         p->pk_off, s));
+  MFP_HIP(hipStreamSynchronize(s));
+  p->ell_packed = true;
+  p->ell_w = w;
+  p->pk_w4 = w4;
+  p->device_bytes = need;
+  return MFP_OK;
+}
+
 // Column-sorted SCOO: slice_size rows per slice (a power of two; a slice
 // accumulates in LDS, 8 bytes per row for either value type).  Rows longer than max(256, mean entries per slice /
 // 128) get their own slices: in an ordinary slice such a row would put many
@@ -549,6 +585,15 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
       const double wd = (double)rs.avg_elements_in_rows +
                         (double)(rs.max_elements_in_rows - rs.avg_elements_in_rows) * t.hyb_percent;
       w = (uint32_t)wd;
+    }
+    if (t.flags & MFP_TUNE_ELL_PACKED) {
+      if (format != MFP_ELL || m->dtype != MFP_F32) {
+        set_error("MFP_TUNE_ELL_PACKED is for ELL plans over f32 values");
+        return fail(MFP_ERR_ARG);
+      }
+      st = plan_create_ell_packed(p, m, w, t, s);
+      if (st != MFP_OK) return fail(st);
+      break;
     }
     const uint64_t slots = (uint64_t)m->rows * w;
     const uint64_t need = slots * (vs + 4);
@@ -663,6 +708,9 @@ static int plan_spmv(mfp_plan p, const void *x, void *y, hipStream_t s, void *pa
     return launch_csr_adaptive(dt, g, m->values, x, y, s);
   }
   case MFP_ELL:
+    if (p->ell_packed)
+      return launch_ell_pk(m->rows, p->ell_w, p->pk_w4, p->pk_base, p->pk_val, p->pk_off, (const float *)x, (float *)y,
+                           s);
     return launch_ell(dt, m->rows, p->ell_w, p->ell_col, p->ell_val, x, y, s);
   case MFP_COO:
     MFP_CHECK(fill_bytes_zero(y, (size_t)m->rows * dtype_size(dt), s));
@@ -796,7 +844,7 @@ int mfp_plan_get_info(mfp_plan p, mfp_plan_info *o)
   o->lane_size = p->L;
   o->lanes = p->t.lanes;
   o->nnz_per_wg = p->t.nnz_per_wg;
-  o->column_bytes = p->off16 ? 2u : 4u;
+  o->column_bytes = p->off16 || p->ell_packed ? 2u : 4u;
   o->scoo_colsort = p->colsort ? 1u : 0u;
   o->scoo_packed = p->colsort ? 1u : 0u;
   o->scoo_long_rows = p->colsort ? p->cs.NL : 0u;
@@ -808,8 +856,8 @@ int mfp_plan_array(mfp_plan p, int which, const void **ptr, uint64_t *count)
 {
   MFP_ARG(p && ptr && count, "null pointer");
   const uint64_t slots = (uint64_t)p->m->rows * p->ell_w;
-  MFP_ARG(!(is_lp(p->dtype) && (which == MFP_ARR_ELL_VALUES || which == MFP_ARR_ELL_COLS)),
-          "16-bit-value ELL plans keep their own packed layout");
+  MFP_ARG(!((is_lp(p->dtype) || p->ell_packed) && (which == MFP_ARR_ELL_VALUES || which == MFP_ARR_ELL_COLS)),
+          "packed ELL plans (16-bit values or MFP_TUNE_ELL_PACKED) keep their own layout");
   switch (which) {
   case MFP_ARR_ELL_VALUES: *ptr = p->ell_val; *count = slots; break;
   case MFP_ARR_ELL_COLS: *ptr = p->ell_col; *count = slots; break;

@@ -543,6 +543,78 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t rows, uint32_t w, const ui
   if (r < rows) ell_row<T, false>(r, rows, w, ec, ev, x, y);
 }
 
+// FP32 ELL over a 64-row sliced layout with 16-bit column offsets (plans
+// created with MFP_TUNE_ELL_PACKED).  Slots 4j .. 4j + 3 of row r are value
+// word j (four floats) and offset word j (four u16), both at
+// ((r / 64) w4 + j) 64 + r % 64 in their arrays; a slot's column is
+// base[r / 64] + its offset.  A slot takes 6 bytes instead of 8, and a wave
+// reads one contiguous run per word instead of one strided run per slot.
+// The fma order is k_ell's, slots 0 .. w - 1 (the slots past w that fill the
+// last word are skipped); a padding slot adds 0 x x[base].
+typedef uint16_t pk_u16x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_ell_pk(uint32_t rows, uint32_t w, uint32_t w4,
+                                                const uint32_t *__restrict__ base, const f32x4 *__restrict__ val,
+                                                const pk_u16x4 *__restrict__ off, const float *__restrict__ x,
+                                                float *__restrict__ y)
+{
+  const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+  if (r >= rows) return;
+  const uint32_t blk = r >> 6, lr = r & 63u;
+  const float *__restrict__ xb = x + base[blk];
+  float d = 0.f;
+  // N words of 4 slots, every load of the group in flight before the first fma
+  auto group = [&](uint32_t j, auto n_const) {
+    constexpr uint32_t N = decltype(n_const)::value;
+    pk_u16x4 o[N];
+    f32x4 v[N];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++) {
+      const uint64_t at = ((uint64_t)blk * w4 + j + q) * 64u + lr;
+      o[q] = __builtin_nontemporal_load(off + at);
+      v[q] = __builtin_nontemporal_load(val + at);
+    }
+    float xv[N][4];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++)
+#pragma unroll
+      for (uint32_t e = 0; e < 4; e++) xv[q][e] = xb[o[q][e]];
+#pragma unroll
+    for (uint32_t q = 0; q < N; q++)
+#pragma unroll
+      for (uint32_t e = 0; e < 4; e++)
+        if (4u * (j + q) + e < w) d = fma_t(v[q][e], xv[q][e], d);
+  };
+  uint32_t j = 0;
+  for (; j + 4 <= w4; j += 4) group(j, std::integral_constant<uint32_t, 4>{});
+  if (j + 2 <= w4) {
+    group(j, std::integral_constant<uint32_t, 2>{});
+    j += 2;
+  }
+  if (j < w4) group(j, std::integral_constant<uint32_t, 1>{});
+  y[r] = d;
+}
+
+// The packed layout of k_ell_pk from CSR: a thread per row (rows rounded up
+// to 64; the extra rows and the slots past a row's length hold value 0,
+// offset 0).
+__global__ void k_ell_pk_build(uint32_t rows, uint32_t stride, const uint32_t *__restrict__ rp,
+                               const uint32_t *__restrict__ ci, const float *__restrict__ a, uint32_t w4,
+                               const uint32_t *__restrict__ base, float *__restrict__ val,
+                               uint16_t *__restrict__ off)
+{
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= stride) return;
+  const uint32_t s = r < rows ? rp[r] : 0u, len = r < rows ? rp[r + 1] - s : 0u;
+  const uint32_t blk = r >> 6, lr = r & 63u, b0 = base[blk];
+  for (uint32_t k = 0; k < 4u * w4; k++) {
+    const bool in = k < len;
+    const uint64_t at = (((uint64_t)blk * w4 + k / 4u) * 64u + lr) * 4u + k % 4u;
+    off[at] = in ? (uint16_t)(ci[s + k] - b0) : (uint16_t)0;
+    val[at] = in ? a[s + k] : 0.f;
+  }
+}
+
 // y += t, element-wise (the host-COO hybrid's merge of the tail sums,
 // gpu_matrix_multiplier.cu:1098-1108)
 template <typename T>
@@ -1177,6 +1249,22 @@ int launch_csr_adaptive(int dtype, const adaptive_args &g, const void *a, const 
 {
   if (dtype == MFP_F32) return adaptive_t<float>(g, (const float *)a, (const float *)x, (float *)y, s);
   return adaptive_t<double>(g, (const double *)a, (const double *)x, (double *)y, s);
+}
+
+int launch_ell_pk(uint32_t rows, uint32_t w, uint32_t w4, const uint32_t *base, const void *val, const uint16_t *off,
+                  const float *x, float *y, hipStream_t s)
+{
+  if (rows == 0) return MFP_OK;
+  return launch_k(k_ell_pk, dim3(ceil_div_u32(rows, 256)), dim3(256), 0, s, rows, w, w4, base, (const f32x4 *)val,
+                  (const pk_u16x4 *)off, x, y);
+}
+
+int ell_pk_build(uint32_t rows, uint32_t stride, const uint32_t *rp, const uint32_t *ci, const float *a, uint32_t w4,
+                 const uint32_t *base, float *val, uint16_t *off, hipStream_t s)
+{
+  if (stride == 0) return MFP_OK;
+  k_ell_pk_build<<<ceil_div_u32(stride, 256), 256, 0, s>>>(rows, stride, rp, ci, a, w4, base, val, off);
+  return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("packed ELL build failed"), MFP_ERR_HIP);
 }
 
 int launch_ell(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const void *ev,

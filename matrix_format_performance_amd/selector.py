@@ -257,6 +257,8 @@ def tuning_label(fmt: str, t: dict, marks: dict, dtype=np.float32) -> str:
             th = "t1024" if t.get("scoo_threads") == 1024 else ""
             return f"cs{t['scoo_slice_size']}p{t.get('scoo_pieces', 0)}{th}"
         return "default" if t["flags"] & L.MFP_TUNE_IS_DEFAULT else f"ss{t['scoo_slice_size']}"
+    if fmt == "ell":
+        return "packed" if t["flags"] & L.MFP_TUNE_ELL_PACKED else "default"
     if fmt in ("hyb", "hyb_atomic"):
         w = t["hyb_width"]
         if w == 0:

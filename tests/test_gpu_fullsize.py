@@ -14,8 +14,8 @@ too.  On top of that, size-independent properties:
 - the 2D Laplacian again with x ~ U[-1, 1) from seed 0, per row against
   the oracle for CSR-scalar, CSR-vector, ELL and COO; the power-law
   matrix with N(0, 1) values stored as BF16 / FP16;
-- 3D 27-point 128^3 (2,097,152 rows, 55.7M nnz): ELL and CSR-vector
-  bit-exact, the column-sorted SCOO within tolerance, x = 1 giving the row
+- 3D 27-point 128^3 (2,097,152 rows, 55.7M nnz): ELL (plain and packed)
+  and CSR-vector bit-exact, the column-sorted SCOO within tolerance, x = 1 giving the row
   sums, and the 16-bit value forms against FP64 of the stored values;
 - the 512-matrix batch (about 20M rows, 430M nnz) for k = 1, 4 and 16:
   sampled members' blocks against the oracle on the member alone, a
@@ -213,7 +213,11 @@ def test_stencil3d_128_bit_exact_and_row_sums(gpu, stencil3d):
     y, info = _spmv(gpu, A, "ell", x, rows)
     w, ed, ec = oracle.ell_from_csr(rp, ci, va)
     assert info["ell_width"] == w == 27
-    assert np.array_equal(y, oracle.spmv_ell(rows, w, ec, ed, x))
+    y_ell = oracle.spmv_ell(rows, w, ec, ed, x)
+    assert np.array_equal(y, y_ell)
+    # the packed FP32 ELL (16-bit offsets, 6 bytes per slot): the same order
+    y, info = _spmv(gpu, A, "ell", x, rows, flags=gpu._lib.MFP_TUNE_ELL_PACKED)
+    assert info["column_bytes"] == 2 and np.array_equal(y, y_ell)
     y, _ = _spmv(gpu, A, "csr_vector", x, rows, lanes=32)
     assert np.array_equal(y, oracle.spmv_lanes(rp, ci, va, x, 32))
     y, _ = _spmv(gpu, A, "scoo", x, rows, **COLSORT)

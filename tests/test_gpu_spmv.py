@@ -113,6 +113,55 @@ def test_ell_bit_exact(gpu, name, dtype):
     assert np.array_equal(y, oracle.spmv_ell(rp.size - 1, w, ec, ed, x))
 
 
+def _block_span(rp, ci):
+    """Largest column span of a 64-row block (last column of its rows minus
+    the smallest first column), or -1 for a matrix without entries."""
+    span = -1
+    for b0 in range(0, rp.size - 1, 64):
+        r = np.arange(b0, min(b0 + 64, rp.size - 1))
+        nz = r[rp[r + 1] > rp[r]]
+        if nz.size:
+            span = max(span, int(ci[rp[nz + 1] - 1].max()) - int(ci[rp[nz]].min()))
+    return span
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_ell_packed_bit_exact(gpu, name):
+    # MFP_TUNE_ELL_PACKED (FP32): 64-row slices, 16-bit offsets, four slots
+    # to a word; the same fma order as the plain ELL, so equal to the
+    # oracle's ELL order bit for bit, or refused when a slice spans 65,536
+    # columns or more
+    rp, ci, va, cols = case(name, "float32")
+    flags = gpu._lib.MFP_TUNE_ELL_PACKED
+    if _block_span(rp, ci) >= 65536:
+        A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+        with pytest.raises(gpu.MfpError, match="span"):
+            gpu.Plan(A, "ell", flags=flags)
+        A.destroy()
+        return
+    y, (rp, ci, va, x), info = _run(gpu, name, "float32", "ell", flags=flags)
+    w, ed, ec = oracle.ell_from_csr(rp, ci, va)
+    assert info["ell_width"] == w and info["column_bytes"] == 2
+    assert np.array_equal(y, oracle.spmv_ell(rp.size - 1, w, ec, ed, x))
+
+
+def test_ell_packed_refusals(gpu):
+    rp, ci, va, cols = case("stencil3d_20", "float64")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    with pytest.raises(gpu.MfpError, match="f32"):
+        gpu.Plan(A, "ell", flags=gpu._lib.MFP_TUNE_ELL_PACKED)
+    A.destroy()
+    rp, ci, va, cols = case("stencil3d_20", "float32")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
+    with pytest.raises(gpu.MfpError, match="f32"):
+        gpu.Plan(A, "hyb", flags=gpu._lib.MFP_TUNE_ELL_PACKED)
+    plan = gpu.Plan(A, "ell", flags=gpu._lib.MFP_TUNE_ELL_PACKED)
+    with pytest.raises(gpu.MfpError, match="own layout"):
+        plan.array(gpu._lib.MFP_ARR_ELL_VALUES, np.float32)
+    plan.destroy()
+    A.destroy()
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("fmt", ["coo", "scoo", "hyb", "hyb_atomic"])
 @pytest.mark.parametrize("name", CASES)
