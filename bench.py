@@ -189,6 +189,8 @@ def tuning_summary(fmt, t):
     out = {k: t[k] for k in keys}
     if fmt == "scoo":
         out["column_sorted"] = bool(t["flags"] & 4)
+    if fmt == "ell":
+        out["packed_16bit_columns"] = bool(t["flags"] & 32)
     return out
 
 
@@ -206,6 +208,8 @@ def kernel_names(fmt, info):
         return ["k_scoo_cs", "k_scoo_cs_finish"] if info["scoo_colsort"] else ["k_scoo"]
     if fmt == "csr_adaptive":
         return ["k_csr_adaptive", "k_adaptive_finish"] if info["chunks"] else ["k_csr_adaptive"]
+    if fmt == "ell" and info["column_bytes"] == 2:
+        return ["k_ell_pk"]
     return {"csr_scalar": ["k_csr_scalar"], "csr_vector": ["k_csr_vector"], "ell": ["k_ell"], "coo": ["k_coo"],
             "hyb": ["k_ell", "k_coo"] if info["coo_elements"] else ["k_ell"],
             "hyb_atomic": ["k_hyb_atomic"]}.get(fmt)
@@ -366,18 +370,27 @@ def precision_study(mfp, timer, flush, reps, cache):
         yd = mfp.DeviceArray(rows, np.float32)
         y64, ad = A.oracle_f64(xd)
         res = {}
-        for vt in ("f32", "bf16", "f16"):
-            M = A if vt == "f32" else A.convert(vt)
-            s_v = 4 if vt == "f32" else 2
+        # "f32_off16": FP32 values in the packed ELL (16-bit column offsets,
+        # 6 bytes per slot), reported beside the plain FP32 forms, which the
+        # 16-bit value forms' speed-up is measured against
+        for vt in ("f32", "f32_off16", "bf16", "f16"):
+            M = A if vt.startswith("f32") else A.convert(vt)
+            s_v = 4 if vt.startswith("f32") else 2
             nb = b_alg(rows, rows, nnz, s_v, 4)
             best = None
-            for fmt, kw in (("csr_vector", {}), ("csr_adaptive", {}), ("ell", {}),
-                            ("scoo", {"flags": mfp._lib.MFP_TUNE_SCOO_COLSORT, "scoo_slice_size": 16384,
-                                      "scoo_threads": 1024})):
+            cands = ((("ell", {"flags": mfp._lib.MFP_TUNE_ELL_PACKED}),) if vt == "f32_off16" else
+                     (("csr_vector", {}), ("csr_adaptive", {}), ("ell", {}),
+                      ("scoo", {"flags": mfp._lib.MFP_TUNE_SCOO_COLSORT, "scoo_slice_size": 16384,
+                                "scoo_threads": 1024})))
+            for fmt, kw in cands:
                 try:
                     p = mfp.Plan(M, fmt, **kw)
                 except mfp.MfpBudgetError:
                     continue
+                except mfp.MfpError:
+                    if vt != "f32_off16":
+                        raise
+                    continue  # the packed layout is refused where a 64-row slice spans too many columns
                 time_plan(mfp, p, xd, yd, 1, flush, timer)
                 ms = statistics.median(time_plan(mfp, p, xd, yd, reps, flush, timer))
                 if best is None or ms < best[1]:
@@ -386,11 +399,13 @@ def precision_study(mfp, timer, flush, reps, cache):
                     best = (fmt, ms, p)
                 else:
                     p.destroy()
+            if best is None:
+                continue
             fmt, ms, p = best
             p.spmv(xd, yd)
             mfp.synchronize()
             total = mfp.residual(yd, y64, ad)
-            if vt == "f32":
+            if vt.startswith("f32"):
                 arith = total
             else:
                 ys, ads = M.oracle_f64(xd)
@@ -409,8 +424,13 @@ def precision_study(mfp, timer, flush, reps, cache):
                     hb = json.load(f)["hbm_bytes_per_launch"]
                 res[vt]["counter_hbm_bytes"] = hb
                 res[vt]["counter_GB_per_s"] = round(hb / (ms * 1e-3) / 1e9, 1)
-            if vt != "f32":
+            if vt == "f32_off16":
+                res[vt]["format"] = "ell, 16-bit column offsets"
                 res[vt]["speedup_vs_f32"] = round(res["f32"]["ms"] / ms, 3)
+            elif vt != "f32":
+                res[vt]["speedup_vs_f32"] = round(res["f32"]["ms"] / ms, 3)
+                if "f32_off16" in res:
+                    res[vt]["speedup_vs_f32_off16"] = round(res["f32_off16"]["ms"] / ms, 3)
                 M.destroy()
             log(f"[bench] precision {wl} {vt}: {fmt} {ms * 1e3:.1f} us "
                 f"err(stored) {arith['rel_l2']:.2e} err(total) {total['rel_l2']:.2e}")
@@ -419,8 +439,9 @@ def precision_study(mfp, timer, flush, reps, cache):
         A.destroy()
         out[wl] = res
     out["note"] = ("effective_GB_per_s = B_alg / time with B_alg = nnz (s_v + 4) + (rows + 1) 4 + x + y; the "
-                   "16-bit sliced ELL stores 2-byte column offsets beside its values, so its effective rate can "
-                   "exceed the HBM rate; counter_GB_per_s (where profiles/ holds the counters) is the HBM traffic "
+                   "16-bit sliced ELL and the packed FP32 ELL (f32_off16) store 2-byte column offsets beside "
+                   "their values, so their effective rates can exceed the HBM rate; speedup_vs_f32 is against "
+                   "the plain FP32 forms (4-byte columns), speedup_vs_f32_off16 against the packed FP32 ELL; counter_GB_per_s (where profiles/ holds the counters) is the HBM traffic "
                    "the FETCH_SIZE / WRITE_SIZE counters measured per launch over the same time")
     return out
 
@@ -654,6 +675,8 @@ def main():
 
     traffic = None
     vtag = args.values or args.dtype
+    if best == "ell" and not args.values and pinfo["column_bytes"] == 2:
+        vtag += "_off16"  # the packed FP32 ELL's own counters
     tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{vtag}_{best}.json")
     if not os.path.exists(tpath) and best == "hyb" and dom_names == ["k_ell"]:
         # a hybrid whose COO part is empty runs the ELL kernel on the ELL layout
