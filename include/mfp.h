@@ -193,11 +193,12 @@ enum {
    * partial slots and runs as one kernel.  Meant for many small matrices
    * in one block-diagonal batch, where slices are many and narrow. */
   MFP_TUNE_SCOO_WHOLE_SLICES = 16,
-  /* FP32 ELL plans: a layout of 64-row slices with 16-bit column offsets from
-   * each slice's first column, four slots to a word (16 bytes of values, 8 of
-   * offsets): 6 bytes per slot instead of 8, summed in the same slot order as
-   * the plain ELL.  Refused (MFP_ERR_ARG) for other value types and when a
-   * slice's columns span 65,536 or more.  Such a plan keeps no column-major
+  /* FP32 and FP64 ELL plans: a layout of 64-row slices with 16-bit column
+   * offsets from each slice's first column, four slots to a word (four
+   * values, 8 bytes of offsets): 6 bytes per slot instead of 8 (FP64: 10
+   * instead of 12), summed in the same slot order as the plain ELL.  Refused
+   * (MFP_ERR_ARG) for 16-bit values and when a slice's columns span 65,536
+   * or more.  Such a plan keeps no column-major
    * arrays, so mfp_plan_array has none to give for it.  No reference
    * counterpart (the reference's ELL is column-major, matrix_converter.cpp:170). */
   MFP_TUNE_ELL_PACKED = 32

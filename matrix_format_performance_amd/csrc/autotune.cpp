@@ -137,7 +137,7 @@ int mfp_autotune_candidates(mfp_matrix m, uint32_t format_mask, mfp_candidate *o
     }
     case MFP_ELL:
       push(c, f, tagged);
-      if (v.dtype == MFP_F32) { // the packed layout (its plan refuses slices whose columns span 65,536 or more)
+      if (v.dtype == MFP_F32 || v.dtype == MFP_F64) { // the packed layout (its plan refuses slices whose columns span 65,536 or more)
         mfp_tuning tp = def;
         tp.flags |= MFP_TUNE_ELL_PACKED;
         push(c, f, tp);

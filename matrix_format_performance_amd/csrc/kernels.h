@@ -35,12 +35,13 @@ int launch_csr_vector(int dtype, uint32_t lanes, uint32_t rows, const uint32_t *
                       const uint32_t *ci, const void *a, const void *x, void *y, hipStream_t s);
 int launch_csr_adaptive(int dtype, const adaptive_args &g, const void *a, const void *x, void *y,
                         hipStream_t s);
-// FP32 ELL in the 64-row sliced layout with 16-bit offsets (spmv.hip k_ell_pk);
-// stride = rows rounded up to 64, base[stride / 64] = each block's first column
-int launch_ell_pk(uint32_t rows, uint32_t w, uint32_t w4, const uint32_t *base, const void *val, const uint16_t *off,
-                  const float *x, float *y, hipStream_t s);
-int ell_pk_build(uint32_t rows, uint32_t stride, const uint32_t *rp, const uint32_t *ci, const float *a, uint32_t w4,
-                 const uint32_t *base, float *val, uint16_t *off, hipStream_t s);
+// FP32 / FP64 ELL in the 64-row sliced layout with 16-bit offsets (spmv.hip
+// k_ell_pk); stride = rows rounded up to 64, base[stride / 64] = each block's
+// first column
+int launch_ell_pk(int dtype, uint32_t rows, uint32_t w, uint32_t w4, const uint32_t *base, const void *val,
+                  const uint16_t *off, const void *x, void *y, hipStream_t s);
+int ell_pk_build(int dtype, uint32_t rows, uint32_t stride, const uint32_t *rp, const uint32_t *ci, const void *a,
+                 uint32_t w4, const uint32_t *base, void *val, uint16_t *off, hipStream_t s);
 int launch_ell(int dtype, uint32_t rows, uint32_t w, const uint32_t *ec, const void *ev,
                const void *x, void *y, hipStream_t s);
 int launch_coo_add(int dtype, uint64_t n, const uint32_t *cr, const uint32_t *cc, const void *cv,

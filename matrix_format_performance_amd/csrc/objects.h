@@ -38,11 +38,11 @@ struct mfp_plan_s {
   bool lp_block = false; // 16-bit offsets from a base per 64-row block
   void *lp_cols = nullptr;
   uint32_t *lp_base = nullptr;
-  // FP32 ELL with MFP_TUNE_ELL_PACKED: values and u16 offsets in 64-row slices,
-  // pk_w4 words of 4 slots per row, offsets from pk_base[row / 64]
+  // FP32 / FP64 ELL with MFP_TUNE_ELL_PACKED: values and u16 offsets in 64-row
+  // slices, pk_w4 words of 4 slots per row, offsets from pk_base[row / 64]
   bool ell_packed = false;
   uint32_t pk_w4 = 0;
-  float *pk_val = nullptr;
+  void *pk_val = nullptr;
   uint16_t *pk_off = nullptr;
   uint32_t *pk_base = nullptr;
   // column-sorted SCOO (MFP_TUNE_SCOO_COLSORT)

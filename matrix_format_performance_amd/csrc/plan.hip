@@ -449,15 +449,16 @@ int mfp_tuning_default(mfp_matrix m, int format, mfp_tuning *t)
   return MFP_OK;
 }
 
-// FP32 ELL with MFP_TUNE_ELL_PACKED (spmv.hip k_ell_pk): 64-row slices, each
-// with a base column (its rows' smallest first column) that every column of
-// the slice lies within 65,536 of; w rounded up to words of 4 slots.
+// FP32 / FP64 ELL with MFP_TUNE_ELL_PACKED (spmv.hip k_ell_pk): 64-row slices,
+// each with a base column (its rows' smallest first column) that every column
+// of the slice lies within 65,536 of; w rounded up to words of 4 slots.
 static int plan_create_ell_packed(mfp_plan p, mfp_matrix_s *m, uint32_t w, const mfp_tuning &t, hipStream_t s)
 {
   const uint32_t stride = ell_lp_stride(m->rows);
   const uint32_t w4 = (w + 3) / 4;
   const uint64_t slots = (uint64_t)stride * 4u * w4;
-  const uint64_t need = slots * 6 + (uint64_t)stride / 64 * 4;
+  const uint64_t vs = dtype_size(m->dtype);
+  const uint64_t need = slots * (vs + 2) + (uint64_t)stride / 64 * 4;
   if (need > budget_of(t)) {
     set_error("packed ELL needs " + std::to_string(need) + " bytes, above the budget");
     return MFP_ERR_BUDGET;
@@ -470,9 +471,9 @@ static int plan_create_ell_packed(mfp_plan p, mfp_matrix_s *m, uint32_t w, const
     set_error("packed ELL: a 64-row slice spans " + std::to_string(span) + " columns, more than 16-bit offsets reach");
     return MFP_ERR_ARG;
   }
-  MFP_CHECK(dmalloc((void **)&p->pk_val, std::max<uint64_t>(slots, 1) * 4));
+  MFP_CHECK(dmalloc(&p->pk_val, std::max<uint64_t>(slots, 1) * vs));
   MFP_CHECK(dmalloc((void **)&p->pk_off, std::max<uint64_t>(slots, 1) * 2));
-  MFP_CHECK(ell_pk_build(m->rows, stride, m->row_ptr, m->columns, (const float *)m->values, w4, p->pk_base, p->pk_val,
+  MFP_CHECK(ell_pk_build(m->dtype, m->rows, stride, m->row_ptr, m->columns, m->values, w4, p->pk_base, p->pk_val,
                          p->pk_off, s));
   MFP_HIP(hipStreamSynchronize(s));
   p->ell_packed = true;
@@ -543,6 +544,10 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
     return code;
   };
   if (is_lp(m->dtype)) {
+    if (t.flags & MFP_TUNE_ELL_PACKED) { // the 16-bit-value ELL has 16-bit offsets of its own
+      set_error("MFP_TUNE_ELL_PACKED is for ELL plans over f32 or f64 values");
+      return fail(MFP_ERR_ARG);
+    }
     st = plan_create_lp(p, m, t, s);
     if (st != MFP_OK) return fail(st);
     *out = p;
@@ -587,8 +592,8 @@ int mfp_plan_create(mfp_plan *out, mfp_matrix m, int format, const mfp_tuning *t
       w = (uint32_t)wd;
     }
     if (t.flags & MFP_TUNE_ELL_PACKED) {
-      if (format != MFP_ELL || m->dtype != MFP_F32) {
-        set_error("MFP_TUNE_ELL_PACKED is for ELL plans over f32 values");
+      if (format != MFP_ELL || (m->dtype != MFP_F32 && m->dtype != MFP_F64)) {
+        set_error("MFP_TUNE_ELL_PACKED is for ELL plans over f32 or f64 values");
         return fail(MFP_ERR_ARG);
       }
       st = plan_create_ell_packed(p, m, w, t, s);
@@ -709,8 +714,7 @@ static int plan_spmv(mfp_plan p, const void *x, void *y, hipStream_t s, void *pa
   }
   case MFP_ELL:
     if (p->ell_packed)
-      return launch_ell_pk(m->rows, p->ell_w, p->pk_w4, p->pk_base, p->pk_val, p->pk_off, (const float *)x, (float *)y,
-                           s);
+      return launch_ell_pk(dt, m->rows, p->ell_w, p->pk_w4, p->pk_base, p->pk_val, p->pk_off, x, y, s);
     return launch_ell(dt, m->rows, p->ell_w, p->ell_col, p->ell_val, x, y, s);
   case MFP_COO:
     MFP_CHECK(fill_bytes_zero(y, (size_t)m->rows * dtype_size(dt), s));

@@ -543,38 +543,64 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t rows, uint32_t w, const ui
   if (r < rows) ell_row<T, false>(r, rows, w, ec, ev, x, y);
 }
 
-// FP32 ELL over a 64-row sliced layout with 16-bit column offsets (plans
-// created with MFP_TUNE_ELL_PACKED).  Slots 4j .. 4j + 3 of row r are value
-// word j (four floats) and offset word j (four u16), both at
-// ((r / 64) w4 + j) 64 + r % 64 in their arrays; a slot's column is
-// base[r / 64] + its offset.  A slot takes 6 bytes instead of 8, and a wave
-// reads one contiguous run per word instead of one strided run per slot.
-// The fma order is k_ell's, slots 0 .. w - 1 (the slots past w that fill the
-// last word are skipped); a padding slot adds 0 x x[base].
+// FP32 / FP64 ELL over a 64-row sliced layout with 16-bit column offsets
+// (plans created with MFP_TUNE_ELL_PACKED).  Slots 4j .. 4j + 3 of row r are
+// value word j (four values) and offset word j (four u16); with
+// u = (r / 64) w4 + j, the offset word is at u 64 + r % 64, and the value
+// word is a float4 at u 64 + r % 64 (FP32) or two double2 halves, slots
+// 4j, 4j + 1 at 2u 64 + r % 64 and 4j + 2, 4j + 3 at (2u + 1) 64 + r % 64
+// (FP64), so every 16-byte load of a wave reads 1 KiB contiguous.  A slot's
+// column is base[r / 64] + its offset.  A slot takes 6 bytes instead of 8
+// (FP64: 10 instead of 12).  The fma order is k_ell's, slots 0 .. w - 1 (the
+// slots past w that fill the last word are skipped); a padding slot adds
+// 0 x x[base].
 typedef uint16_t pk_u16x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <typename T> struct pk_word;
+template <> struct pk_word<float> {
+  typedef f32x4 type;
+  typedef f32x4 unit; // the array's element: one word
+  static __device__ __forceinline__ type load(const unit *__restrict__ v, uint64_t u, uint32_t lr)
+  {
+    return __builtin_nontemporal_load(v + u * 64u + lr);
+  }
+};
+template <> struct pk_word<double> {
+  typedef f64x4 type;
+  typedef f64x2 unit; // the array's element: half a word
+  static __device__ __forceinline__ type load(const unit *__restrict__ v, uint64_t u, uint32_t lr)
+  {
+    const f64x2 lo = __builtin_nontemporal_load(v + u * 128u + lr);
+    const f64x2 hi = __builtin_nontemporal_load(v + u * 128u + 64u + lr);
+    return type{lo.x, lo.y, hi.x, hi.y};
+  }
+};
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_ell_pk(uint32_t rows, uint32_t w, uint32_t w4,
-                                                const uint32_t *__restrict__ base, const f32x4 *__restrict__ val,
-                                                const pk_u16x4 *__restrict__ off, const float *__restrict__ x,
-                                                float *__restrict__ y)
+                                                const uint32_t *__restrict__ base,
+                                                const typename pk_word<T>::unit *__restrict__ val,
+                                                const pk_u16x4 *__restrict__ off, const T *__restrict__ x,
+                                                T *__restrict__ y)
 {
+  typedef typename pk_word<T>::type V;
   const uint32_t r = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
   if (r >= rows) return;
   const uint32_t blk = r >> 6, lr = r & 63u;
-  const float *__restrict__ xb = x + base[blk];
-  float d = 0.f;
+  const T *__restrict__ xb = x + base[blk];
+  T d = 0;
   // N words of 4 slots, every load of the group in flight before the first fma
   auto group = [&](uint32_t j, auto n_const) {
     constexpr uint32_t N = decltype(n_const)::value;
     pk_u16x4 o[N];
-    f32x4 v[N];
+    V v[N];
 #pragma unroll
     for (uint32_t q = 0; q < N; q++) {
-      const uint64_t at = ((uint64_t)blk * w4 + j + q) * 64u + lr;
-      o[q] = __builtin_nontemporal_load(off + at);
-      v[q] = __builtin_nontemporal_load(val + at);
+      const uint64_t u = (uint64_t)blk * w4 + j + q;
+      o[q] = __builtin_nontemporal_load(off + u * 64u + lr);
+      v[q] = pk_word<T>::load(val, u, lr);
     }
-    float xv[N][4];
+    T xv[N][4];
 #pragma unroll
     for (uint32_t q = 0; q < N; q++)
 #pragma unroll
@@ -598,9 +624,10 @@ __global__ __launch_bounds__(256) void k_ell_pk(uint32_t rows, uint32_t w, uint3
 // The packed layout of k_ell_pk from CSR: a thread per row (rows rounded up
 // to 64; the extra rows and the slots past a row's length hold value 0,
 // offset 0).
+template <typename T>
 __global__ void k_ell_pk_build(uint32_t rows, uint32_t stride, const uint32_t *__restrict__ rp,
-                               const uint32_t *__restrict__ ci, const float *__restrict__ a, uint32_t w4,
-                               const uint32_t *__restrict__ base, float *__restrict__ val,
+                               const uint32_t *__restrict__ ci, const T *__restrict__ a, uint32_t w4,
+                               const uint32_t *__restrict__ base, T *__restrict__ val,
                                uint16_t *__restrict__ off)
 {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -609,9 +636,12 @@ __global__ void k_ell_pk_build(uint32_t rows, uint32_t stride, const uint32_t *_
   const uint32_t blk = r >> 6, lr = r & 63u, b0 = base[blk];
   for (uint32_t k = 0; k < 4u * w4; k++) {
     const bool in = k < len;
-    const uint64_t at = (((uint64_t)blk * w4 + k / 4u) * 64u + lr) * 4u + k % 4u;
-    off[at] = in ? (uint16_t)(ci[s + k] - b0) : (uint16_t)0;
-    val[at] = in ? a[s + k] : 0.f;
+    const uint64_t u = (uint64_t)blk * w4 + k / 4u;
+    off[(u * 64u + lr) * 4u + k % 4u] = in ? (uint16_t)(ci[s + k] - b0) : (uint16_t)0;
+    // FP32: word u of row lr is one float4; FP64: two double2 halves
+    const uint64_t at = sizeof(T) == 4 ? (u * 64u + lr) * 4u + k % 4u
+                                       : ((u * 2u + (k % 4u) / 2u) * 64u + lr) * 2u + k % 2u;
+    val[at] = in ? a[s + k] : (T)0;
   }
 }
 
@@ -1251,19 +1281,27 @@ int launch_csr_adaptive(int dtype, const adaptive_args &g, const void *a, const 
   return adaptive_t<double>(g, (const double *)a, (const double *)x, (double *)y, s);
 }
 
-int launch_ell_pk(uint32_t rows, uint32_t w, uint32_t w4, const uint32_t *base, const void *val, const uint16_t *off,
-                  const float *x, float *y, hipStream_t s)
+int launch_ell_pk(int dtype, uint32_t rows, uint32_t w, uint32_t w4, const uint32_t *base, const void *val,
+                  const uint16_t *off, const void *x, void *y, hipStream_t s)
 {
   if (rows == 0) return MFP_OK;
-  return launch_k(k_ell_pk, dim3(ceil_div_u32(rows, 256)), dim3(256), 0, s, rows, w, w4, base, (const f32x4 *)val,
-                  (const pk_u16x4 *)off, x, y);
+  const dim3 grid(ceil_div_u32(rows, 256));
+  if (dtype == MFP_F32)
+    return launch_k(k_ell_pk<float>, grid, dim3(256), 0, s, rows, w, w4, base, (const f32x4 *)val,
+                    (const pk_u16x4 *)off, (const float *)x, (float *)y);
+  return launch_k(k_ell_pk<double>, grid, dim3(256), 0, s, rows, w, w4, base, (const f64x2 *)val,
+                  (const pk_u16x4 *)off, (const double *)x, (double *)y);
 }
 
-int ell_pk_build(uint32_t rows, uint32_t stride, const uint32_t *rp, const uint32_t *ci, const float *a, uint32_t w4,
-                 const uint32_t *base, float *val, uint16_t *off, hipStream_t s)
+int ell_pk_build(int dtype, uint32_t rows, uint32_t stride, const uint32_t *rp, const uint32_t *ci, const void *a,
+                 uint32_t w4, const uint32_t *base, void *val, uint16_t *off, hipStream_t s)
 {
   if (stride == 0) return MFP_OK;
-  k_ell_pk_build<<<ceil_div_u32(stride, 256), 256, 0, s>>>(rows, stride, rp, ci, a, w4, base, val, off);
+  const uint32_t g = ceil_div_u32(stride, 256);
+  if (dtype == MFP_F32)
+    k_ell_pk_build<float><<<g, 256, 0, s>>>(rows, stride, rp, ci, (const float *)a, w4, base, (float *)val, off);
+  else
+    k_ell_pk_build<double><<<g, 256, 0, s>>>(rows, stride, rp, ci, (const double *)a, w4, base, (double *)val, off);
   return hipGetLastError() == hipSuccess ? MFP_OK : (set_error("packed ELL build failed"), MFP_ERR_HIP);
 }
 

@@ -228,6 +228,25 @@ def test_stencil3d_128_bit_exact_and_row_sums(gpu, stencil3d):
     _tol(y, rp, ci, va, ones)
 
 
+def test_stencil3d_128_f64_ell_plain_and_packed_bit_exact(gpu, stencil3d):
+    # FP64 values of the same 128^3 stencil: the plain ELL (12 bytes per slot)
+    # and the packed one (10 bytes per slot) both add slots 0 .. 26 in the
+    # oracle's ELL order
+    rp, ci, va32, _ = stencil3d
+    va = va32.astype(np.float64)
+    rows = rp.size - 1
+    announce("full size stencil3d_128 f64 ell plain and packed")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, rows)
+    x = gpu.generate_vector(3, rows, np.float64)
+    w, ed, ec = oracle.ell_from_csr(rp, ci, va)
+    y_ell = oracle.spmv_ell(rows, w, ec, ed, x)
+    y, info = _spmv(gpu, A, "ell", x, rows)
+    assert info["ell_width"] == 27 and np.array_equal(y, y_ell)
+    y, info = _spmv(gpu, A, "ell", x, rows, flags=gpu._lib.MFP_TUNE_ELL_PACKED)
+    assert info["column_bytes"] == 2 and np.array_equal(y, y_ell)
+    A.destroy()
+
+
 def test_batch512_member_blocks_and_checksum(gpu):
     # the 512-matrix batch (about 20M rows, 430M nnz) as the benchmark runs
     # it: k = 1 through the column-sorted SCOO, k = 4 and 16 through the

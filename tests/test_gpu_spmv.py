@@ -125,13 +125,14 @@ def _block_span(rp, ci):
     return span
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("name", CASES)
-def test_ell_packed_bit_exact(gpu, name):
-    # MFP_TUNE_ELL_PACKED (FP32): 64-row slices, 16-bit offsets, four slots
-    # to a word; the same fma order as the plain ELL, so equal to the
+def test_ell_packed_bit_exact(gpu, name, dtype):
+    # MFP_TUNE_ELL_PACKED (FP32 / FP64): 64-row slices, 16-bit offsets, four
+    # slots to a word; the same fma order as the plain ELL, so equal to the
     # oracle's ELL order bit for bit, or refused when a slice spans 65,536
     # columns or more
-    rp, ci, va, cols = case(name, "float32")
+    rp, ci, va, cols = case(name, dtype)
     flags = gpu._lib.MFP_TUNE_ELL_PACKED
     if _block_span(rp, ci) >= 65536:
         A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
@@ -139,16 +140,16 @@ def test_ell_packed_bit_exact(gpu, name):
             gpu.Plan(A, "ell", flags=flags)
         A.destroy()
         return
-    y, (rp, ci, va, x), info = _run(gpu, name, "float32", "ell", flags=flags)
+    y, (rp, ci, va, x), info = _run(gpu, name, dtype, "ell", flags=flags)
     w, ed, ec = oracle.ell_from_csr(rp, ci, va)
     assert info["ell_width"] == w and info["column_bytes"] == 2
     assert np.array_equal(y, oracle.spmv_ell(rp.size - 1, w, ec, ed, x))
 
 
 def test_ell_packed_refusals(gpu):
-    rp, ci, va, cols = case("stencil3d_20", "float64")
-    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols)
-    with pytest.raises(gpu.MfpError, match="f32"):
+    rp, ci, va, cols = case("stencil3d_20", "float32")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, cols).convert("bf16")
+    with pytest.raises(gpu.MfpError, match="f32 or f64"):
         gpu.Plan(A, "ell", flags=gpu._lib.MFP_TUNE_ELL_PACKED)
     A.destroy()
     rp, ci, va, cols = case("stencil3d_20", "float32")
