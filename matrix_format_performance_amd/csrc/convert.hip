@@ -676,8 +676,13 @@ __global__ void k_cs_scatter(uint64_t nnz, const uint32_t *__restrict__ perm, co
     // then holds the block's positions l, 64 + l, 128 + l and 192 + l, so
     // each of the wave's four gathers reads 64 consecutive columns
     const uint64_t b0 = i & ~(uint64_t)255;
-    const uint64_t d = (b0 >= items[lo].y && b0 + 256 <= items[lo].z) ? b0 + 4 * (i & 63) + ((i >> 6) & 3) : i;
-    ev[d] = a[src];
+    const bool in_block = b0 >= items[lo].y && b0 + 256 <= items[lo].z;
+    const uint64_t d = in_block ? b0 + 4 * (i & 63) + ((i >> 6) & 3) : i;
+    // FP64 values of such a block: slots 0 and 1 of its 64 words first, as
+    // 64 consecutive 16-byte pairs, then slots 2 and 3, so that each 16-byte
+    // value load of a wave reads 1 KiB contiguous
+    const uint64_t dv = sizeof(T) == 8 && in_block ? b0 + (((i >> 7) & 1) << 7) + 2 * (i & 63) + ((i >> 6) & 1) : d;
+    ev[dv] = a[src];
     ec[d] = ((c - base[lo]) << rbits) | lrow[src];
   }
 }

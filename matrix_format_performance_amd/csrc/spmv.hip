@@ -847,8 +847,11 @@ struct cs_values {
 #pragma unroll
       for (uint32_t j = 0; j < 4; j++) v[j] = (double)f[j];
     } else {
-      const f64x2 d0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(ev) + 2 * g);
-      const f64x2 d1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(ev) + 2 * g + 1);
+      // FP64: slots 0, 1 of the block's 64 words, then slots 2, 3 (word g
+      // is word g % 64 of block g / 64)
+      const f64x2 *p = reinterpret_cast<const f64x2 *>(ev) + (g >> 6) * 128u + (g & 63u);
+      const f64x2 d0 = __builtin_nontemporal_load(p);
+      const f64x2 d1 = __builtin_nontemporal_load(p + 64);
       v[0] = d0[0];
       v[1] = d0[1];
       v[2] = d1[0];
