@@ -342,6 +342,38 @@ def test_scoo_colsort_long_rows_fill_several_slices(gpu, dtype, P):
     _tol_check(y, rp, ci, va, x)
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("R,P,flags", [(256, 1, 0), (256, 4, 0), (1024, 0, 0), (4096, 0, 16)])
+def test_scoo_colsort_integer_values_exact(gpu, dtype, R, P, flags):
+    # values and x small integers: every product, partial slot and row sum
+    # is exact in FP32 and FP64 whatever the order, so y equals the integer
+    # row sums bit for bit; a value read from the wrong slot or half of a
+    # 16-byte entry word, or a lost entry, changes them
+    rng = np.random.default_rng(21)
+    rows, cols = 6000, 20000
+    lens = rng.integers(0, 13, rows)
+    lens[rng.choice(rows, 40, replace=False)] = rng.integers(300, 2000, 40)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    ci = np.concatenate([np.sort(rng.choice(cols, n, replace=False)) for n in lens]).astype(np.uint32)
+    vi = rng.integers(-8, 9, ci.size)
+    xi = rng.integers(-8, 9, cols)
+    want = np.array([int(np.dot(vi[rp[r]:rp[r + 1]], xi[ci[rp[r]:rp[r + 1]]])) for r in range(rows)])
+    dt = np.dtype(dtype)
+    announce(f"scoo colsort integer values dtype={dtype} R={R} P={P} flags={flags}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, vi.astype(dt), cols)
+    plan = gpu.Plan(A, "scoo", flags=gpu._lib.MFP_TUNE_SCOO_COLSORT | flags, scoo_slice_size=R, scoo_pieces=P)
+    xd = gpu.DeviceArray.from_host(xi.astype(dt))
+    yd = gpu.DeviceArray(rows, dt)
+    yd.fill(np.nan)
+    plan.spmv(xd, yd)
+    gpu.synchronize()
+    info = plan.info()
+    assert info["scoo_colsort"] == 1
+    assert np.array_equal(yd.to_host(), want.astype(dt))
+    plan.destroy()
+    A.destroy()
+
+
 @pytest.mark.parametrize("whole", [False, True])
 @pytest.mark.parametrize("vt", ["bf16", "f16"])
 def test_scoo_colsort_16bit_values_in_the_last_band(gpu, vt, whole):
