@@ -8,7 +8,9 @@ from timings this package measures (``sweep``: every autotuner candidate of
 every format, per matrix), and it infers in FP32 or in BF16 on the matrix
 cores.  The pick is the most probable format; its tuning comes from rules
 fitted on the same timings (per bin of mean row length and row count, the
-tuning with the lowest mean slowdown within the format).  The exhaustive
+tuning with the lowest mean slowdown within the format; a label that lets
+the plan size its own work items ranks ahead of a fixed piece count unless
+that one wins by more than ``SELF_SIZED_MARGIN``).  The exhaustive
 autotuner stays the fallback and the yardstick for the pick's regret.
 
 Command line (``python -m matrix_format_performance_amd.selector``):
@@ -20,11 +22,16 @@ Command line (``python -m matrix_format_performance_amd.selector``):
 * ``train --data S.jsonl [T.jsonl ...] --out W.npz [--hold-every 5]``: holds
   out every fifth record, trains on the others, fits the tuning rules, saves
   both (``Selector.load``) and reports regret on the held-out records.
+  The data files may be gzipped (``.jsonl.gz``).
+* ``refit-rules --data S.jsonl [...] --out W.npz [--weights V.npz]``: keeps
+  the network of V (the shipped one by default), fits the tuning rules again
+  on the same training split, saves both and reports as ``train`` does.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
+import gzip
 import json
 import math
 import os
@@ -324,10 +331,27 @@ def tuning_keys(feats) -> list:
     return [f"{m},{r},{s}", f"{m},{r}", f"{m}", ""]
 
 
-def fit_tuning_rules(records, min_count: int = 3) -> dict:
+# A label whose plan sizes its own work items from the matrix and the CU
+# count (the column-sorted SCOO with its pieces chosen by the plan, "p0")
+# ranks ahead of a fixed piece count unless that one is better by more than
+# this mean slowdown.  A fixed count fits the training matrices of a key but
+# not every matrix in it: on the 1M-row power-law matrix 2 pieces per
+# 16,384-row slice leave 128 work items for 256 CUs (53.2 against 42.4 us),
+# while 6 training matrices of its key ranked 2 pieces first by 0.2%.
+# Chosen on a validation split of the training records (the margins 0 to
+# 0.15 tried; 0.1 and above gave the lowest validation regret).
+SELF_SIZED_MARGIN = 0.1
+
+
+def _self_sized(label: str) -> bool:
+    return label.startswith("cs") and "p0" in label
+
+
+def fit_tuning_rules(records, min_count: int = 3, self_sized_margin: float = SELF_SIZED_MARGIN) -> dict:
     """For every format and key, tuning labels ordered by their mean slowdown
     against the format's best tuning on the training matrices (a label that
-    failed to build counts as 10x slower)."""
+    failed to build counts as 10x slower), self-sized labels credited with
+    self_sized_margin."""
     rules = {}
     for f in FORMAT_ORDER:
         agg = {}
@@ -346,7 +370,9 @@ def fit_tuning_rules(records, min_count: int = 3) -> dict:
             n = max(len(v) for v in labels.values())
             if n < min_count and key != "":
                 continue
-            table[key] = [lab for lab, _ in sorted(labels.items(), key=lambda kv: (float(np.mean(kv[1])), kv[0]))]
+            table[key] = [lab for lab, _ in sorted(
+                labels.items(),
+                key=lambda kv: (float(np.mean(kv[1])) - (self_sized_margin if _self_sized(kv[0]) else 0.0), kv[0]))]
         rules[f] = table
     return rules
 
@@ -471,7 +497,7 @@ def sweep(out: str, count: int, seed: int = 0, max_nnz: int = 1 << 24, reps: int
 
 def load_records(path: str) -> list:
     recs = {}
-    with open(path) as f:
+    with (gzip.open(path, "rt") if path.endswith(".gz") else open(path)) as f:
         for line in f:
             try:
                 r = json.loads(line)
@@ -517,9 +543,10 @@ def fixed_format_baseline(train_records, test_records, rules=None) -> dict:
             "geomean_slowdown": float(np.exp(np.mean(np.log1p(rg)))), "within_5pct": float(np.mean(rg <= 0.05))}
 
 
-def train_selector(records, tau=0.05, epochs=300, lr=2e-3, batch=128, seed=0, log=None):
+def train_selector(records, tau=0.05, epochs=300, lr=2e-3, batch=128, seed=0, log=None,
+                   self_sized_margin=SELF_SIZED_MARGIN):
     """Rules from the records, then the network on rule-based targets."""
-    rules = fit_tuning_rules(records)
+    rules = fit_tuning_rules(records, self_sized_margin=self_sized_margin)
     X, P = dataset(records, rules, tau)
     net = train(X, P, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
     return Selector(net, rules)
@@ -548,6 +575,15 @@ def main(argv=None) -> int:
     t.add_argument("--report", default=None, help="also write the evaluation JSON here")
     t.add_argument("--tau", type=float, default=0.05, help="target temperature (relative slowdown)")
     t.add_argument("--tune", action="store_true", help="choose tau and epochs on a validation split")
+    t.add_argument("--self-sized-margin", type=float, default=SELF_SIZED_MARGIN)
+    f = sub.add_parser("refit-rules", help="fit a trained selector's tuning rules again on sweep records, "
+                                           "keeping its network, and evaluate it as train does")
+    f.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    f.add_argument("--data", required=True, nargs="+", help="sweep JSONL files (records joined in order)")
+    f.add_argument("--out", required=True)
+    f.add_argument("--hold-every", type=int, default=5)
+    f.add_argument("--self-sized-margin", type=float, default=SELF_SIZED_MARGIN)
+    f.add_argument("--report", default=None, help="also write the evaluation JSON here")
     a = ap.parse_args(argv)
     E.init()
     if a.cmd == "sweep":
@@ -560,6 +596,16 @@ def main(argv=None) -> int:
     tr = [r for i, r in enumerate(recs) if not (k and i % k == k - 1)]
     te = [r for i, r in enumerate(recs) if k and i % k == k - 1]
     log = lambda m: print(m, flush=True)  # noqa: E731
+    if a.cmd == "refit-rules":
+        sl = Selector.load(a.weights)
+        sl.rules = fit_tuning_rules(tr, self_sized_margin=a.self_sized_margin)
+        sl.save(a.out)
+        cfg = {"refit_rules_of": os.path.basename(a.weights), "train": len(tr), "held_out": len(te),
+               "self_sized_margin": a.self_sized_margin}
+        if a.report and os.path.exists(a.report):
+            with open(a.report) as fh:  # the network's own training settings, from its report
+                cfg["network"] = json.load(fh).get("config")
+        return _report(a, sl, tr, te, cfg, [])
     # hyper-parameters on a validation split of the training matrices
     grid = [(tau, ep) for tau in (0.02, 0.05, 0.1, 0.2) for ep in (150, 300, 600)] if a.tune else [(a.tau, a.epochs)]
     choice, tuning_log = grid[0], []
@@ -577,10 +623,16 @@ def main(argv=None) -> int:
             if ev["mean_regret"] < best:
                 best, choice = ev["mean_regret"], (tau, ep)
     tau, ep = choice
-    sl = train_selector(tr, tau, ep, a.lr, seed=a.seed, log=log)
+    sl = train_selector(tr, tau, ep, a.lr, seed=a.seed, log=log, self_sized_margin=a.self_sized_margin)
     sl.save(a.out)
+    return _report(a, sl, tr, te, {"tau": tau, "epochs": ep, "lr": a.lr, "train": len(tr), "held_out": len(te),
+                                   "self_sized_margin": a.self_sized_margin}, tuning_log)
+
+
+def _report(a, sl, tr, te, config, tuning_log) -> int:
+    """The evaluation JSON of a selector on its training and held-out records."""
     Xtr = np.array([r["features"] for r in tr], np.float32)
-    rep = {"config": {"tau": tau, "epochs": ep, "lr": a.lr, "train": len(tr), "held_out": len(te)},
+    rep = {"config": config,
            "validation": tuning_log,
            "train_with_rules": evaluate_with_rules(tr, sl.net.predict(Xtr, bf16=True), sl.rules)}
     if te:

@@ -81,3 +81,18 @@ def test_rules_split_by_column_spread():
     f = [0.0] * 64
     f[3], f[0], f[58] = 3.5 / 16, 20.2 / 32, 0.5
     assert sel.choose_label(rules, "scoo", f, {"cs4096p0", "cs16384p0"}) == "cs16384p0"
+
+
+def test_rules_prefer_plan_sized_pieces_unless_a_fixed_count_wins_clearly():
+    # the column-sorted SCOO with its pieces chosen by the plan (p0) ranks
+    # ahead of a fixed piece count that is better by less than the margin
+    # (a near tie), and behind one that is better by more
+    def recs(fixed_ms):
+        return [_rec(3.5 / 16, 20.2 / 32, [["scoo", "cs16384p0t1024", 1.0, True],
+                                         ["scoo", "cs16384p2t1024", fixed_ms, False]]) for _ in range(4)]
+    near = sel.fit_tuning_rules(recs(0.998))
+    assert near["scoo"]["3,20"][:2] == ["cs16384p0t1024", "cs16384p2t1024"]
+    assert sel.fit_tuning_rules(recs(0.998), self_sized_margin=0.0)["scoo"]["3,20"][0] == "cs16384p2t1024"
+    clear = sel.fit_tuning_rules(recs(0.8))  # p0 is 25% slower than p2
+    assert clear["scoo"]["3,20"][0] == "cs16384p2t1024"
+    assert sel.SELF_SIZED_MARGIN == 0.1
