@@ -198,8 +198,8 @@ def tuning_summary(fmt, t):
     out = {k: t[k] for k in keys}
     if fmt == "scoo":
         out["column_sorted"] = bool(t["flags"] & 4)
-    if fmt == "ell":
-        out["packed_16bit_columns"] = bool(t["flags"] & 32)
+    if fmt == "ell" and t["flags"] & 32:
+        out["packed_16bit_columns"] = True  # MFP_TUNE_ELL_PACKED (16-bit value plans have a layout of their own)
     return out
 
 
