@@ -612,20 +612,18 @@ __global__ __launch_bounds__(256) void k_ell_pk(uint32_t rows, uint32_t w, uint3
         if (4u * (j + q) + e < w) d = fma_t(v[q][e], xv[q][e], d);
   };
   uint32_t j = 0;
-  // FP32: a row of 5 to 8 words (up to 32 slots) is one group, every load of
-  // the row in flight before the first gather (3D 27-point stencil, 7 words:
-  // 62.2 against 64.0 us in groups of 4, 2 and 1); FP64 words hold twice the
-  // registers and keep the groups below
-  if constexpr (sizeof(T) == 4) {
-    if (w4 >= 5 && w4 <= 8) {
-      switch (w4) {
-      case 8: group(0, std::integral_constant<uint32_t, 8>{}); break;
-      case 7: group(0, std::integral_constant<uint32_t, 7>{}); break;
-      case 6: group(0, std::integral_constant<uint32_t, 6>{}); break;
-      default: group(0, std::integral_constant<uint32_t, 5>{}); break;
-      }
-      j = w4;
+  // a row of 5 to 8 words (up to 32 slots) is one group, every load of the
+  // row in flight before the first gather (3D 27-point stencil, 7 words:
+  // FP32 62.2 against 64.0 us in groups of 4, 2 and 1, FP64 111.5 against
+  // 113.5 us, though FP64 then holds 148 VGPRs, 3 waves per SIMD)
+  if (w4 >= 5 && w4 <= 8) {
+    switch (w4) {
+    case 8: group(0, std::integral_constant<uint32_t, 8>{}); break;
+    case 7: group(0, std::integral_constant<uint32_t, 7>{}); break;
+    case 6: group(0, std::integral_constant<uint32_t, 6>{}); break;
+    default: group(0, std::integral_constant<uint32_t, 5>{}); break;
     }
+    j = w4;
   }
   for (; j + 4 <= w4; j += 4) group(j, std::integral_constant<uint32_t, 4>{});
   if (j + 2 <= w4) {
