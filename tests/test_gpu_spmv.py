@@ -565,3 +565,69 @@ def test_plan_kernel_times_bf16_binned(gpu):
     plan.destroy()
     B.destroy()
     A.destroy()
+
+
+def _banded(max_len, rows=3000, seed=31):
+    # rows of 1 .. max_len entries (at least one of max_len) within 150
+    # columns of the diagonal: every 64-row slice spans far fewer than 65,536
+    rng = np.random.default_rng(seed + max_len)
+    lens = rng.integers(1, max_len + 1, rows)
+    lens[rng.integers(0, rows)] = max_len
+    cols_ = []
+    for r, n in enumerate(lens):
+        lo, hi = max(0, r - 150), min(rows, r + 150)
+        cols_.append(np.sort(rng.choice(np.arange(lo, hi), n, replace=False)))
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    ci = np.concatenate(cols_).astype(np.uint32)
+    return rp, ci, rng.standard_normal(ci.size)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("max_len", [12, 16, 17, 21, 24, 29, 32, 33])
+def test_ell_packed_row_words_bit_exact(gpu, dtype, max_len):
+    # rows of 3 to 9 four-slot words: those of 5 to 8 words are read as one
+    # group, the others in groups of 4, 2 and 1; every grouping adds the
+    # slots in the oracle's ELL order
+    rp, ci, va = _banded(max_len)
+    va = va.astype(np.dtype(dtype))
+    rows = rp.size - 1
+    announce(f"ell packed words max_len={max_len} dtype={dtype}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, rows)
+    plan = gpu.Plan(A, "ell", flags=gpu._lib.MFP_TUNE_ELL_PACKED)
+    x = gpu.generate_vector(5, rows, np.dtype(dtype))
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(rows, np.dtype(dtype))
+    yd.fill(np.nan)
+    plan.spmv(xd, yd)
+    gpu.synchronize()
+    w, ed, ec = oracle.ell_from_csr(rp, ci, va)
+    assert plan.info()["ell_width"] == w == max_len
+    assert np.array_equal(yd.to_host(), oracle.spmv_ell(rows, w, ec, ed, x))
+    plan.destroy()
+    A.destroy()
+
+
+@pytest.mark.parametrize("vt", ["bf16", "f16"])
+@pytest.mark.parametrize("max_len", [12, 16, 17, 21, 24, 29, 32, 33])
+def test_ell_16bit_row_words_compensated(gpu, vt, max_len):
+    # the 16-bit sliced ELL over rows of 3 to 9 words (one group up to 8):
+    # Dot2 keeps the error of the stored-value products near one FP32 rounding
+    rp, ci, va = _banded(max_len)
+    va = va.astype(np.float32)
+    rows = rp.size - 1
+    announce(f"ell 16-bit words max_len={max_len} vt={vt}")
+    A = gpu.DeviceMatrix.from_csr(rp, ci, va, rows).convert(vt)
+    plan = gpu.Plan(A, "ell")
+    assert plan.info()["column_bytes"] == 2
+    x = gpu.generate_vector(6, rows, np.float32)
+    xd = gpu.DeviceArray.from_host(x)
+    yd = gpu.DeviceArray(rows, np.float32)
+    yd.fill(np.nan)
+    plan.spmv(xd, yd)
+    gpu.synchronize()
+    from oracle.selector import bf16_round
+    stored = bf16_round(va) if vt == "bf16" else va.astype(np.float16).astype(np.float32)
+    y64, ad = oracle.spmv_f64(rp, ci, stored, x)
+    assert oracle.tolerance_check(yd.to_host(), y64, ad, 1.0) <= 2.5e-7
+    plan.destroy()
+    A.destroy()
