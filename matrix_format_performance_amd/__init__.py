@@ -18,7 +18,16 @@ from .reference_api import (compare_results, coo_matrix_class, csr_matrix_class,
                             gpu_hybrid_spmv, gpu_scoo_spmv, hybrid_matrix_class, matrix_class,
                             matrix_meta, matrix_rows_statistic, measurement_class,
                             resizable_gpu_memory, scoo_geometry, scoo_matrix_class)
-from . import selector  # noqa: E402
+
+
+def __getattr__(name):
+    # the selector module loads on first use, so that
+    # ``python -m matrix_format_performance_amd.selector`` runs it only once
+    if name == "selector":
+        import importlib
+        return importlib.import_module(".selector", __name__)
+    raise AttributeError(f"module {__name__!r} has no attribute {name!r}")
+
 
 __all__ = [
     "MfpError", "MfpBudgetError", "FORMATS", "LABELS", "Batch", "autotune", "DeviceArray", "DeviceMatrix",
